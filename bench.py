@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark: plain single SumCheck proofs per second on one MI355X.
+
+Workload (one step = one whole proof, results copied to the host):
+  a*b*c + d*e over 5 uniform tables from the seeded counter-based generator, SHA3 transcript
+  derived on the device, Goldilocks (2^64 - 2^32 + 1) at num_vars = 26 (headline), plus the
+  other three widths at their sizes (2^32-5 at 26, 2^127-1 and BLS12-381 Fr at 24) reported
+  under config.per_width.
+
+Prints ONE JSON line. `roofline` is for the dominant kernel (the fused fold+eval pass
+kernel): algorithmic bytes of every pass launch in the timed region divided by their summed
+durations, both measured here with hipEvents recorded on the stream the kernels run on.
+`cpu_baseline` times the oracle's single-core Python restatement of the reference prover
+(src/zkduel/reference.py:44-110) on a bounded sample and scales it to the full size.
+
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu] [--widths 64 256]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path[:0] = [str(ROOT / "sumcheck-triton_amd"), str(ROOT)]
+
+METRIC = "SumCheck proofs/s + % HBM/INT32 roofline; batched vs plain; p50/p99 latency"
+HBM_PEAK_GBPS = 8000.0
+SIZES = {32: 26, 64: 26, 128: 24, 256: 24}
+WORKLOAD = "poly_abc_plus_de"
+SEED = 2026
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pass_bytes(T: int, limbs: int, n0: int, j: int) -> int:
+    """Algorithmic HBM bytes of round j's pass: read round j-1's tables (or the input at
+    j = 0) once, write round j's folded tables once."""
+    w = 4 * limbs
+    if j == 0:
+        return T * w * n0
+    return T * w * ((n0 >> (j - 1)) + (n0 >> j))
+
+
+def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int) -> dict:
+    from zkduel_hip import Composition, Proof, get_field, get_spec
+    from zkduel_hip.driver import algorithmic_bytes
+
+    f = get_field(bw)
+    spec = get_spec(WORKLOAD)
+    comp = Composition.from_spec(spec)
+    T = spec.num_vars
+    with Proof(ctx, bw, comp, nv, input="uniform", seed=SEED) as pr:
+        pr.set_profiling(True)
+        for _ in range(warmup):
+            pr.run("sha3")
+            pr.wait()
+        lat, pass_ms_sum, pass_bytes_sum, pass_launches, dev_ms = [], 0.0, 0, 0, []
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s0 = time.perf_counter()
+            pr.run("sha3")
+            dev_ms.append(pr.wait())
+            pr.result()
+            lat.append((time.perf_counter() - s0) * 1000.0)
+            passes, _tail = pr.kernel_times()
+            for j, ms in enumerate(passes):
+                pass_ms_sum += ms
+                pass_bytes_sum += pass_bytes(T, f.limbs, 1 << nv, j)
+                pass_launches += 1
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        info = pr.info()
+    lat.sort()
+    achieved = pass_bytes_sum / (pass_ms_sum * 1e6) if pass_ms_sum > 0 else 0.0
+    algo = algorithmic_bytes(T, f.limbs, nv)
+    return {
+        "bit_width": bw,
+        "num_vars": nv,
+        "proofs_per_s": steps / wall,
+        "ms_per_proof": wall * 1000.0 / steps,
+        "device_ms_median": statistics.median(dev_ms),
+        "p50_ms": statistics.median(lat),
+        "p99_ms": lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))],
+        "algo_bytes_per_proof": algo,
+        "proof_hbm_gbps": algo / (statistics.median(dev_ms) * 1e6),
+        "pass_kernel_gbps": achieved,
+        "pass_kernel_avg_ms": pass_ms_sum / max(pass_launches, 1),
+        "pass_kernel_avg_bytes": pass_bytes_sum / max(pass_launches, 1),
+        "pass_rounds": info["pass_rounds"],
+        "hbm_frac": achieved / HBM_PEAK_GBPS,
+        "wall_s": wall,
+    }
+
+
+def cpu_baseline(budget_s: float = 20.0) -> dict:
+    """Oracle (single-core Python restatement of reference.py) on the same workload at the
+    largest num_vars that fits the budget; proofs/s scaled to num_vars 26 by 2^(nv - 26)
+    (the prover's work is linear in the table size)."""
+    from oracle import sumcheck_ref as R
+    from zkduel_hip import get_spec
+
+    p, L = R.field(64)
+    spec = get_spec(WORKLOAD)
+    terms = [(c, list(spec.term_vars[o : o + n])) for c, n, o in zip(spec.coeffs, spec.term_lens, spec.term_offsets)]
+    nv, secs = 8, 0.0
+    while True:
+        tables = R.uniform_tables(spec.num_vars, nv, SEED, p, L)
+        t0 = time.perf_counter()
+        R.prove(tables, terms, p, nv, R.sha3_challenger(p, L))
+        secs = time.perf_counter() - t0
+        if secs * 2.2 > budget_s or nv >= 20:
+            break
+        nv += 1
+    full = SIZES[64]
+    return {
+        "value": (1.0 / secs) * 2.0 ** (nv - full),
+        "unit": "proofs/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"oracle/sumcheck_ref.py prove(), Goldilocks a*b*c+d*e, 5 uniform tables, SHA3, "
+                  f"num_vars={nv} took {secs:.2f} s on 1 core; scaled by 2^({nv}-{full})",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--widths", type=int, nargs="+", default=[64, 32, 128, 256])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(instances shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
+        return 2
+    if 64 not in args.widths:
+        args.widths = [64] + args.widths
+
+    from zkduel_hip import Context
+
+    ctx = Context(0)
+    per = {}
+    for bw in args.widths:
+        steps = args.steps if bw == 64 else max(3, args.steps // 2)
+        log(f"bench: width {bw} num_vars {SIZES[bw]} steps {steps}")
+        per[bw] = measure_width(ctx, bw, SIZES[bw], steps, args.warmup)
+        log(f"bench: width {bw}: {per[bw]['proofs_per_s']:.1f} proofs/s, "
+            f"pass kernel {per[bw]['pass_kernel_gbps']:.0f} GB/s")
+    head = per[64]
+    cpu = None
+    if not args.no_cpu:
+        log("bench: cpu baseline")
+        cpu = cpu_baseline()
+    line = {
+        "metric": METRIC,
+        "value": round(head["proofs_per_s"], 3),
+        "unit": "proofs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_proof"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic: seeded counter-based uniform tables in [0,p), SHA3 transcript challenges",
+        "config": {
+            "workload": "plain single SumCheck, a*b*c + d*e over 5 uniform tables, Goldilocks, "
+                        "num_vars=26, degree 3, SHA3 transcript on device",
+            "bit_width": 64,
+            "num_vars": 26,
+            "tables": 5,
+            "parallelism": "single GPU",
+            "p50_ms": round(head["p50_ms"], 4),
+            "p99_ms": round(head["p99_ms"], 4),
+            "per_width": {
+                str(bw): {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+                for bw, r in per.items()
+            },
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(head["pass_kernel_gbps"], 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(head["pass_kernel_gbps"] / HBM_PEAK_GBPS, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

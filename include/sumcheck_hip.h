@@ -1,0 +1,142 @@
+/* C ABI of libsumcheck_hip.so, the MI355X (gfx950) SumCheck prover.
+ *
+ * It replaces the reference's CUDA executable build/field_sumcheck_cuda
+ * (cuda/src/field_sumcheck.cu), which the reference's Python drivers run as a subprocess
+ * (benchmarks/field_sweep/compare_triton_cuda.py:108-136,
+ * experiments/h100/tools/validate_cuda_vs_registry.py:66-91), and the in-process Triton round
+ * loop (benchmarks/field_sweep/triton_sweep.py:521-816). Python binds it with ctypes; plain
+ * pointers and sizes only.
+ *
+ * Conventions
+ *   - Every function returns SC_OK (0) or a nonzero SC_ERR_* code; sc_last_error() then
+ *     describes the failure (thread-local). The reference calls std::exit instead
+ *     (cuda/src/field_sumcheck.cu:18-26, 1112-1173).
+ *   - Field elements are little-endian arrays of L 32-bit limbs, L = bit_width / 32, for
+ *     bit_width 32 (p = 2^32-5), 64 (Goldilocks), 128 (2^127-1), 256 (BLS12-381 scalar)
+ *     (src/zkduel/fields.py:258-294). "Montgomery" means x*2^(32L) mod p.
+ *   - Tables are element-major: table v, entry i at words [(v*2^num_vars + i)*L, +L).
+ *   - The caller owns host buffers; a context owns device buffers it allocates.
+ *   - Calls on one context are serialized by the caller; contexts are independent.
+ */
+#ifndef SUMCHECK_HIP_H
+#define SUMCHECK_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SC_OK 0
+#define SC_ERR_ARG 1         /* invalid argument (ValueError on the Python side) */
+#define SC_ERR_UNSUPPORTED 2 /* outside the compiled limits (ValueError) */
+#define SC_ERR_HIP 3         /* HIP runtime failure (RuntimeError) */
+#define SC_ERR_MEMORY 4      /* device allocation failed (MemoryError) */
+
+/* Composition sum_k coeff_k * prod_m table[term_vars[term_offsets[k] + m]],
+ * m < term_lens[k]: the reference's WorkloadSpec encoding
+ * (benchmarks/field_sweep/workload_specs.py:6-17, src/zkduel/_expr.py:87-110). Coefficients are
+ * canonical field elements in [0, p) as L limbs each (negative integers are taken mod p, as
+ * reference.py:17-20 does). Degree d = max term length <= 7 (cuda/src/field_sumcheck.cu:29);
+ * num_tables <= 32; num_terms <= 32. */
+typedef struct sc_composition {
+  int32_t num_tables;
+  int32_t num_terms;
+  const int32_t* term_lens;    /* [num_terms] */
+  const int32_t* term_offsets; /* [num_terms] */
+  const int32_t* term_vars;    /* [sum of term_lens], values in [0, num_tables) */
+  const uint32_t* coeffs;      /* [num_terms][L] canonical */
+} sc_composition;
+
+#define SC_INPUT_HOST_CANONICAL 0 /* data: host tables, canonical values in [0, p) */
+#define SC_INPUT_HOST_MONT 1      /* data: host tables, Montgomery form */
+#define SC_INPUT_DEVICE_MONT 2    /* data: device tables, Montgomery form, borrowed */
+#define SC_INPUT_AFFINE 3         /* generated: seed + 7919(v+1) + 104729(o+1) (field_sumcheck.cu:123-128) */
+#define SC_INPUT_UNIFORM 4        /* generated: counter-based uniform in [0,p); selector tables in {0,1} */
+
+typedef struct sc_input {
+  int32_t kind;
+  uint32_t instance;      /* UNIFORM: instance number in the generator key */
+  const void* data;       /* HOST_* / DEVICE_MONT */
+  uint64_t seed;          /* AFFINE: the value seed (the reference passes seed + rounds); UNIFORM: key seed */
+  uint64_t selector_mask; /* UNIFORM: bit v set -> table v drawn from {0, 1} */
+} sc_input;
+
+#define SC_CHALLENGE_FIXED 0    /* r_j = 1 (cuda/src/field_sumcheck.cu:1058) */
+#define SC_CHALLENGE_SHA3 1     /* SHA3-256 transcript (field_sumcheck.cu:857-907, triton_sweep.py:474-484) */
+#define SC_CHALLENGE_EXPLICIT 2 /* caller's list, canonical (reference.py:44-55 `challenges`) */
+
+typedef struct sc_context sc_context;
+typedef struct sc_proof sc_proof;
+
+const char* sc_last_error(void);
+const char* sc_version(void);
+
+int sc_context_create(int device, sc_context** out);
+int sc_context_destroy(sc_context* ctx);
+int sc_context_sync(sc_context* ctx);
+
+/* Allocates a proof's device buffers and prepares its input (generation, upload, Montgomery
+ * encode): the reference's per-case setup (field_sumcheck.cu:1178-1221). num_vars in [1, 40]. */
+int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* comp,
+                    int32_t num_vars, const sc_input* input, sc_proof** out);
+
+/* Tunables of the round schedule: pass grid cap and the pair count at or below which the
+ * remaining rounds run in the single-workgroup tail kernel. 0 keeps the default. */
+int sc_proof_set_schedule(sc_proof* proof, int32_t max_pass_blocks, int64_t tail_pairs);
+
+/* Enqueues one whole proof (all rounds, transcript on the device) on the context's stream
+ * and returns without waiting: the reference's run_once (field_sumcheck.cu:1012-1088).
+ * explicit_r: [num_vars-1][L] canonical, only for SC_CHALLENGE_EXPLICIT (else NULL). */
+int sc_proof_run(sc_proof* proof, int32_t challenge_mode, const uint32_t* explicit_r);
+
+/* Captures the same sequence into a hipGraph; later sc_proof_run calls with the same mode
+ * replay it. */
+int sc_proof_capture(sc_proof* proof, int32_t challenge_mode, const uint32_t* explicit_r);
+
+/* Waits for the enqueued runs; *ms = device time of the last run (hipEvents), may be NULL. */
+int sc_proof_wait(sc_proof* proof, float* ms);
+
+/* Results of the last run: round_polys [num_vars][degree+1][L] Montgomery g_j(0..d);
+ * challenges [num_vars-1][L] Montgomery r_j; checksum = sum over rounds, points and limbs of
+ * (i+1)*limb_i (field_sumcheck.cu:1043-1049). Any output pointer may be NULL. */
+int sc_proof_result(sc_proof* proof, uint32_t* round_polys, uint32_t* challenges,
+                    uint64_t* checksum);
+
+/* Copies the prepared input tables (Montgomery, element-major) to host memory. */
+int sc_proof_tables(sc_proof* proof, uint32_t* out);
+
+/* Describes the prepared proof: degree, number of rounds done as fused passes and device
+ * bytes held. Any pointer may be NULL. */
+int sc_proof_info(sc_proof* proof, int32_t* degree, int32_t* pass_rounds, uint64_t* device_bytes);
+
+/* Per-kernel timing: when enabled, eager runs record a hipEvent pair around every pass
+ * kernel and around the tail kernel, on the stream they run on. sc_proof_kernel_times then
+ * returns, for the last run, pass_ms[j] for rounds j < pass_rounds and *tail_ms. */
+int sc_proof_set_profiling(sc_proof* proof, int32_t enable);
+int sc_proof_kernel_times(sc_proof* proof, float* pass_ms, int32_t capacity, float* tail_ms);
+
+int sc_proof_destroy(sc_proof* proof);
+
+/* One-shot: create, run, wait, read results, destroy. */
+int sc_prove(sc_context* ctx, int32_t bit_width, const sc_composition* comp, int32_t num_vars,
+             const sc_input* input, int32_t challenge_mode, const uint32_t* explicit_r,
+             uint32_t* round_polys, uint32_t* challenges, uint64_t* checksum);
+
+/* Elementwise device field operations on n elements (host buffers, [n][L]):
+ * op 0: a*b*R^-1 (Montgomery product), 1: a+b, 2: a-b, 3: a -> a*R (encode),
+ * 4: a -> a*R^-1 (decode). The device twin of src/zkduel/fields.py:63-115, and what the
+ * reference's tests/test_uint256.py exercises. b may be NULL for ops 3 and 4. */
+int sc_field_op(sc_context* ctx, int32_t bit_width, int32_t op, uint64_t n, const uint32_t* a,
+                const uint32_t* b, uint32_t* out);
+
+/* Device transcript challenge for one round: g_mont [npts][L] Montgomery limbs in;
+ * r_mont [L] out (field_sumcheck.cu:857-907 computed on the GPU). */
+int sc_transcript_challenge(sc_context* ctx, int32_t bit_width, int32_t round, const uint32_t* g_mont,
+                            int32_t npts, uint32_t* r_mont);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SUMCHECK_HIP_H */

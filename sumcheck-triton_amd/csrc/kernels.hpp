@@ -1,0 +1,447 @@
+// HIP kernels of the SumCheck prover for one field policy F (instantiated once per field in
+// inst_f*.hip). Reference semantics: cuda/src/field_sumcheck.cu:593-694 (fill, eval, reduce,
+// fold) and :1012-1088 (round loop); src/zkduel/reference.py:78-108.
+//
+// One round j is one `pass_kernel` launch plus one single-block `finalize_kernel` launch:
+//   pass_kernel     each thread owns pairs i of round j. For j >= 1 it reads entries
+//                   4i..4i+3 of round j-1's tables, folds them with r_{j-1} into round j's
+//                   entries 2i, 2i+1 (written back for round j+1), and evaluates the
+//                   composition at the d+1 points t = 0..d of that pair, accumulating
+//                   g_j(t) in registers. So each round streams the tables once (fused
+//                   fold(j-1) + eval(j)); round 0 only reads. Block partial sums go to HBM.
+//   finalize_kernel sums the block partials, stores g_j, and derives r_j on the device
+//                   (SHA3 transcript, fixed r = 1, or an explicit list), so the host never
+//                   waits between rounds and a whole proof can be captured in a hipGraph.
+// Late rounds whose tables are small run in `tail_kernel`: one workgroup loops over the
+// remaining rounds with the same per-pair code, reducing and hashing in-kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "field.hpp"
+#include "keccak.hpp"
+#include "layout.hpp"
+#include "prng.hpp"
+
+namespace sc {
+
+// ---------------------------------------------------------------------------------------------
+// Vector memory helpers (element-major: NW consecutive words per thread)
+// ---------------------------------------------------------------------------------------------
+template <int NW>
+__device__ __forceinline__ void load_words(uint32_t (&w)[NW], const uint32_t* __restrict__ p) {
+  if constexpr (NW % 4 == 0) {
+#pragma unroll
+    for (int i = 0; i < NW / 4; ++i) {
+      const uint4 x = reinterpret_cast<const uint4*>(p)[i];
+      w[4 * i + 0] = x.x;
+      w[4 * i + 1] = x.y;
+      w[4 * i + 2] = x.z;
+      w[4 * i + 3] = x.w;
+    }
+  } else {
+    static_assert(NW == 2, "pairs of 1-limb entries");
+    const uint2 x = *reinterpret_cast<const uint2*>(p);
+    w[0] = x.x;
+    w[1] = x.y;
+  }
+}
+
+template <int NW>
+__device__ __forceinline__ void store_words(uint32_t* __restrict__ p, const uint32_t (&w)[NW]) {
+  if constexpr (NW % 4 == 0) {
+#pragma unroll
+    for (int i = 0; i < NW / 4; ++i) {
+      reinterpret_cast<uint4*>(p)[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+    }
+  } else {
+    static_assert(NW == 2, "pairs of 1-limb entries");
+    *reinterpret_cast<uint2*>(p) = make_uint2(w[0], w[1]);
+  }
+}
+
+template <int L>
+__device__ __forceinline__ Elem<L> elem_at(const uint32_t* w, int k) {
+  Elem<L> x;
+#pragma unroll
+  for (int i = 0; i < L; ++i) x.v[i] = w[k * L + i];
+  return x;
+}
+
+template <int L>
+__device__ __forceinline__ Elem<L> load_elem(const uint32_t* p) {
+  Elem<L> x;
+#pragma unroll
+  for (int i = 0; i < L; ++i) x.v[i] = p[i];
+  return x;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Composition and per-pair evaluation
+// ---------------------------------------------------------------------------------------------
+
+// C(x) = sum_k c_k prod_v x_v^e_kv. Term structure is wave-uniform, so every branch below is
+// a scalar branch; table slots are indexed statically (unrolled v), keeping x in VGPRs.
+template <class F, int TMAX>
+__device__ __forceinline__ void compose_add(Elem<F::L>& acc, const Elem<F::L> (&x)[TMAX],
+                                            const CompDesc* __restrict__ cd, int K) {
+  constexpr int L = F::L;
+  for (int k = 0; k < K; ++k) {
+    Elem<L> prod = zero<F>();
+    bool have = false;
+#pragma unroll
+    for (int v = 0; v < TMAX; ++v) {
+      uint32_t e = (cd->exps[k][v >> 3] >> ((v & 7) * 4)) & 15u;
+      for (; e != 0u; --e) {
+        if (have) {
+          prod = mont_mul<F>(prod, x[v]);
+        } else {
+          prod = x[v];
+          have = true;
+        }
+      }
+    }
+    const int kind = cd->kind[k];
+    if (kind == COEFF_ONE) {
+      acc = add<F>(acc, prod);
+    } else if (kind == COEFF_MINUS_ONE) {
+      acc = sub<F>(acc, prod);
+    } else {
+      acc = add<F>(acc, mont_mul<F>(prod, load_elem<L>(cd->coeff[k])));
+    }
+  }
+}
+
+// cur holds f[2i] and oth holds f[2i+1] of every table on entry. Accumulates
+// C(f[2i] + t (f[2i+1] - f[2i])) into acc[t] for t < npts, with t >= 2 reached by repeated
+// addition of the difference (reference.py:89-99).
+template <class F, int TMAX, int PMAX>
+__device__ __forceinline__ void eval_points(Elem<F::L> (&cur)[TMAX], Elem<F::L> (&oth)[TMAX],
+                                            Elem<F::L> (&acc)[PMAX], const CompDesc* __restrict__ cd,
+                                            int T, int K, int npts) {
+  compose_add<F, TMAX>(acc[0], cur, cd, K);
+#pragma unroll
+  for (int v = 0; v < TMAX; ++v) {
+    if (v < T) {
+      const Elem<F::L> o = oth[v];
+      oth[v] = sub<F>(o, cur[v]);
+      cur[v] = o;
+    }
+  }
+  compose_add<F, TMAX>(acc[1], cur, cd, K);
+#pragma unroll
+  for (int t = 2; t < PMAX; ++t) {
+    if (t < npts) {
+#pragma unroll
+      for (int v = 0; v < TMAX; ++v) {
+        if (v < T) cur[v] = add<F>(cur[v], oth[v]);
+      }
+      compose_add<F, TMAX>(acc[t], cur, cd, K);
+    }
+  }
+}
+
+// Loads pair i of the current round for every table, folding from the previous round's
+// tables when fold != 0 (f' = e + r (o - e), field_sumcheck.cu:677-694), and writes the
+// folded entries to dst.
+template <class F, int TMAX>
+__device__ __forceinline__ void load_pair(Elem<F::L> (&cur)[TMAX], Elem<F::L> (&oth)[TMAX],
+                                          const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                          uint64_t src_n, uint64_t i, int T, bool fold,
+                                          const Elem<F::L>& r) {
+  constexpr int L = F::L;
+#pragma unroll
+  for (int v = 0; v < TMAX; ++v) {
+    if (v < T) {
+      if (fold) {
+        uint32_t w[4 * L];
+        load_words<4 * L>(w, src + ((uint64_t)v * src_n + 4 * i) * L);
+        const Elem<L> f0 = elem_at<L>(w, 0), f1 = elem_at<L>(w, 1);
+        const Elem<L> f2 = elem_at<L>(w, 2), f3 = elem_at<L>(w, 3);
+        const Elem<L> e = add<F>(f0, mont_mul<F>(r, sub<F>(f1, f0)));
+        const Elem<L> o = add<F>(f2, mont_mul<F>(r, sub<F>(f3, f2)));
+        uint32_t y[2 * L];
+#pragma unroll
+        for (int k = 0; k < L; ++k) {
+          y[k] = e.v[k];
+          y[L + k] = o.v[k];
+        }
+        store_words<2 * L>(dst + ((uint64_t)v * (src_n >> 1) + 2 * i) * L, y);
+        cur[v] = e;
+        oth[v] = o;
+      } else {
+        uint32_t w[2 * L];
+        load_words<2 * L>(w, src + ((uint64_t)v * src_n + 2 * i) * L);
+        cur[v] = elem_at<L>(w, 0);
+        oth[v] = elem_at<L>(w, 1);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Reductions (exact modular sums, so any order gives the same bits)
+// ---------------------------------------------------------------------------------------------
+
+// Sums acc over the block into out[t*L + limb] (LDS) for t < npts. red needs
+// (BLOCK/64)*PMAX*L words. Ends with a barrier.
+template <class F, int PMAX>
+__device__ __forceinline__ void block_reduce(Elem<F::L> (&acc)[PMAX], int npts, uint32_t* red,
+                                             uint32_t* out) {
+  constexpr int L = F::L;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int t = 0; t < PMAX; ++t) {
+    if (t < npts) {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        Elem<L> o;
+#pragma unroll
+        for (int k = 0; k < L; ++k) o.v[k] = __shfl_xor(acc[t].v[k], off, 64);
+        acc[t] = add<F>(acc[t], o);
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < L; ++k) red[(wave * PMAX + t) * L + k] = acc[t].v[k];
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < npts) {
+    const int t = threadIdx.x;
+    Elem<L> s = load_elem<L>(red + t * L);
+#pragma unroll
+    for (int w = 1; w < BLOCK / 64; ++w) s = add<F>(s, load_elem<L>(red + (w * PMAX + t) * L));
+#pragma unroll
+    for (int k = 0; k < L; ++k) out[t * L + k] = s.v[k];
+  }
+  __syncthreads();
+}
+
+// SHA3-256 of a padded message held in LDS, run by one whole wave on wave-uniform values.
+__device__ __forceinline__ void sha3_uniform(const uint32_t* msg, int nblocks, uint32_t digest[8]) {
+  uint64_t A[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) A[i] = 0;
+  for (int b = 0; b < nblocks; ++b) {
+#pragma unroll
+    for (int i = 0; i < 17; ++i) {
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)msg[34 * b + 2 * i]);
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)msg[34 * b + 2 * i + 1]);
+      A[i] ^= (uint64_t)lo | ((uint64_t)hi << 32);
+    }
+    keccak_f1600(A);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    digest[2 * i] = (uint32_t)A[i];
+    digest[2 * i + 1] = (uint32_t)(A[i] >> 32);
+  }
+}
+
+// Publishes g (LDS, npts*L words) as round `round`'s polynomial and, unless it is the last
+// round, derives r_round into r_out (LDS, L words) and st->r[round]. All threads call it.
+template <class F>
+__device__ void finish_round(const uint32_t* g, int npts, int round, int num_rounds, int mode,
+                             const FieldConsts* __restrict__ fc, ProofState* st,
+                             const uint32_t* __restrict__ explicit_r, uint32_t* msg, uint32_t* r_out) {
+  constexpr int L = F::L;
+  const int tid = threadIdx.x;
+  if (tid < npts * L) st->polys[round][tid / L][tid % L] = g[tid];
+  if (round + 1 >= num_rounds) return;
+  if (mode == CHALLENGE_SHA3) {
+    // Digest of the transcript, read as a little-endian integer, reduced mod p and
+    // Montgomery-encoded (field_sumcheck.cu:857-907): sum_i digest_i * 2^(32i) * R mod p.
+    const int nvals = npts * L;
+    const int nb = transcript_blocks(nvals);
+    for (int k = tid; k < 34 * nb; k += BLOCK) msg[k] = transcript_word(k, (uint32_t)round, g, nvals, nb);
+    __syncthreads();
+    if (tid < 64) {
+      uint32_t digest[8];
+      sha3_uniform(msg, nb, digest);
+      Elem<L> r = zero<F>();
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        Elem<L> w = zero<F>();
+        w.v[0] = digest[i];
+        r = add<F>(r, mont_mul<F>(w, load_elem<L>(fc->word_pow[i])));
+      }
+      if (tid == 0) {
+#pragma unroll
+        for (int k = 0; k < L; ++k) r_out[k] = r.v[k];
+      }
+    }
+  } else if (mode == CHALLENGE_FIXED) {
+    if (tid < L) r_out[tid] = fc->one[tid];  // r = 1 (field_sumcheck.cu:1058)
+  } else {
+    if (tid == 0) {
+      const Elem<L> x = load_elem<L>(explicit_r + (uint64_t)round * L);
+      const Elem<L> r = mont_mul<F>(x, load_elem<L>(fc->r2));
+#pragma unroll
+      for (int k = 0; k < L; ++k) r_out[k] = r.v[k];
+    }
+  }
+  __syncthreads();
+  if (tid < L) st->r[round][tid] = r_out[tid];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------------------------
+
+template <class F, int TMAX, int PMAX>
+__global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
+  constexpr int L = F::L;
+  __shared__ uint32_t s_red[(BLOCK / 64) * PMAX * L];
+  __shared__ uint32_t s_g[PMAX * L];
+  const CompDesc* __restrict__ cd = a.comp;
+  const int T = cd->num_tables;
+  const int K = cd->num_terms;
+  const int npts = cd->npts;
+  const bool fold = a.fold != 0;
+
+  Elem<L> acc[PMAX];
+#pragma unroll
+  for (int t = 0; t < PMAX; ++t) acc[t] = zero<F>();
+  Elem<L> r = zero<F>();
+  if (fold) r = load_elem<L>(a.st->r[a.round - 1]);
+
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.pairs; i += stride) {
+    Elem<L> cur[TMAX], oth[TMAX];
+    load_pair<F, TMAX>(cur, oth, a.src, a.dst, a.src_n, i, T, fold, r);
+    eval_points<F, TMAX, PMAX>(cur, oth, acc, cd, T, K, npts);
+  }
+  block_reduce<F, PMAX>(acc, npts, s_red, s_g);
+  if (threadIdx.x < npts * L) {
+    a.partials[(uint64_t)blockIdx.x * (MAX_POINTS * L) + threadIdx.x] = s_g[threadIdx.x];
+  }
+}
+
+
+
+template <class F, int PMAX>
+__global__ void __launch_bounds__(BLOCK) finalize_kernel(FinalizeArgs a) {
+  constexpr int L = F::L;
+  __shared__ uint32_t s_red[(BLOCK / 64) * PMAX * L];
+  __shared__ uint32_t s_g[PMAX * L];
+  __shared__ uint32_t s_msg[3 * 34];
+  __shared__ uint32_t s_r[L];
+  const int npts = a.comp->npts;
+  Elem<L> acc[PMAX];
+#pragma unroll
+  for (int t = 0; t < PMAX; ++t) acc[t] = zero<F>();
+  for (uint32_t b = threadIdx.x; b < a.nblocks; b += BLOCK) {
+#pragma unroll
+    for (int t = 0; t < PMAX; ++t) {
+      if (t < npts) acc[t] = add<F>(acc[t], load_elem<L>(a.partials + ((uint64_t)b * MAX_POINTS + t) * L));
+    }
+  }
+  block_reduce<F, PMAX>(acc, npts, s_red, s_g);
+  finish_round<F>(s_g, npts, a.round, a.num_rounds, a.challenge_mode, a.fc, a.st, a.explicit_r, s_msg, s_r);
+}
+
+template <class F, int TMAX, int PMAX>
+__global__ void __launch_bounds__(BLOCK) tail_kernel(TailArgs a) {
+  constexpr int L = F::L;
+  __shared__ uint32_t s_red[(BLOCK / 64) * PMAX * L];
+  __shared__ uint32_t s_g[PMAX * L];
+  __shared__ uint32_t s_msg[3 * 34];
+  __shared__ uint32_t s_r[L];
+  const CompDesc* __restrict__ cd = a.comp;
+  const int T = cd->num_tables;
+  const int K = cd->num_terms;
+  const int npts = cd->npts;
+
+  if (a.first_round > 0 && threadIdx.x < L) s_r[threadIdx.x] = a.st->r[a.first_round - 1][threadIdx.x];
+  __syncthreads();
+  for (int j = a.first_round; j < a.num_rounds; ++j) {
+    const bool fold = j > 0;
+    const uint32_t* src = j <= 1 ? a.input : (((j - 1) & 1) ? a.buf_a : a.buf_b);
+    uint32_t* dst = (j & 1) ? a.buf_a : a.buf_b;
+    const uint64_t src_n = fold ? (a.n0 >> (j - 1)) : a.n0;
+    const uint64_t pairs = a.n0 >> (j + 1);
+    const Elem<L> r = load_elem<L>(s_r);
+    Elem<L> acc[PMAX];
+#pragma unroll
+    for (int t = 0; t < PMAX; ++t) acc[t] = zero<F>();
+    for (uint64_t i = threadIdx.x; i < pairs; i += BLOCK) {
+      Elem<L> cur[TMAX], oth[TMAX];
+      load_pair<F, TMAX>(cur, oth, src, dst, src_n, i, T, fold, r);
+      eval_points<F, TMAX, PMAX>(cur, oth, acc, cd, T, K, npts);
+    }
+    block_reduce<F, PMAX>(acc, npts, s_red, s_g);
+    finish_round<F>(s_g, npts, j, a.num_rounds, a.challenge_mode, a.fc, a.st, a.explicit_r, s_msg, s_r);
+    __syncthreads();
+  }
+}
+
+// Input generation and Montgomery encoding (cuda/src/field_sumcheck.cu:593-611).
+
+template <class F>
+__global__ void __launch_bounds__(BLOCK) fill_kernel(FillArgs a) {
+  constexpr int L = F::L;
+  const uint64_t total = a.n * (uint64_t)a.num_tables;
+  const Elem<L> r2 = load_elem<L>(a.fc->r2);
+  for (uint64_t idx = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; idx < total;
+       idx += (uint64_t)gridDim.x * BLOCK) {
+    const uint32_t v = (uint32_t)(idx / a.n);
+    const uint64_t o = idx - (uint64_t)v * a.n;
+    Elem<L> x;
+    if (a.kind == FILL_AFFINE) {
+      x = affine_element<F>(a.seed, v, o);
+    } else if ((a.selector_mask >> v) & 1u) {
+      x = selector_element<F>(stream_key(a.seed, a.instance, v), o);
+    } else {
+      x = uniform_element<F>(stream_key(a.seed, a.instance, v), o);
+    }
+    const Elem<L> m = mont_mul<F>(x, r2);
+#pragma unroll
+    for (int k = 0; k < L; ++k) a.tables[idx * L + k] = m.v[k];
+  }
+}
+
+// Elementwise field operations, for the field-layer tests and host data conversion.
+enum FieldOp : int32_t { OP_MUL = 0, OP_ADD = 1, OP_SUB = 2, OP_TO_MONT = 3, OP_FROM_MONT = 4 };
+
+template <class F>
+__global__ void __launch_bounds__(BLOCK) field_op_kernel(int op, uint64_t n, const uint32_t* a,
+                                                        const uint32_t* b, uint32_t* out,
+                                                        const FieldConsts* fc) {
+  constexpr int L = F::L;
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK) {
+    const Elem<L> x = load_elem<L>(a + i * L);
+    Elem<L> y;
+    if (op == OP_MUL) {
+      y = mont_mul<F>(x, load_elem<L>(b + i * L));
+    } else if (op == OP_ADD) {
+      y = add<F>(x, load_elem<L>(b + i * L));
+    } else if (op == OP_SUB) {
+      y = sub<F>(x, load_elem<L>(b + i * L));
+    } else if (op == OP_TO_MONT) {
+      y = mont_mul<F>(x, load_elem<L>(fc->r2));
+    } else {
+      Elem<L> one = zero<F>();
+      one.v[0] = 1u;
+      y = mont_mul<F>(x, one);
+    }
+#pragma unroll
+    for (int k = 0; k < L; ++k) out[i * L + k] = y.v[k];
+  }
+}
+
+// Derives one transcript challenge from g (Montgomery limbs) on the device.
+template <class F>
+__global__ void __launch_bounds__(BLOCK) transcript_kernel(const uint32_t* g, int npts, int round,
+                                                          const FieldConsts* fc, ProofState* st) {
+  constexpr int L = F::L;
+  __shared__ uint32_t s_g[MAX_POINTS * L];
+  __shared__ uint32_t s_msg[3 * 34];
+  __shared__ uint32_t s_r[L];
+  if (threadIdx.x < npts * L) s_g[threadIdx.x] = g[threadIdx.x];
+  __syncthreads();
+  finish_round<F>(s_g, npts, round, round + 2, CHALLENGE_SHA3, fc, st, nullptr, s_msg, s_r);
+}
+
+}  // namespace sc

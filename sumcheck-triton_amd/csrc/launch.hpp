@@ -1,0 +1,33 @@
+// Per-field kernel launchers, defined in inst_f*.hip (one translation unit per field so the
+// four fields compile in parallel) and called by the host runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "layout.hpp"
+
+namespace sc {
+
+struct Launchers {
+  int limbs;
+  hipError_t (*fill)(hipStream_t, const FillArgs&);
+  hipError_t (*pass)(hipStream_t, int tmax, int pmax, const PassArgs&, unsigned grid);
+  hipError_t (*finalize)(hipStream_t, int pmax, const FinalizeArgs&);
+  hipError_t (*tail)(hipStream_t, int tmax, int pmax, const TailArgs&);
+  hipError_t (*field_op)(hipStream_t, int op, uint64_t n, const uint32_t* a, const uint32_t* b,
+                         uint32_t* out, const FieldConsts* fc);
+  hipError_t (*transcript)(hipStream_t, const uint32_t* g, int npts, int round,
+                           const FieldConsts* fc, ProofState* st);
+};
+
+const Launchers& launchers_p32();
+const Launchers& launchers_p64();
+const Launchers& launchers_p128();
+const Launchers& launchers_p256();
+
+// Table-slot capacities compiled for the per-pair kernels (the smallest that holds the
+// composition's tables is used) and point capacities (d+1 <= 4 or <= 8).
+constexpr int TMAX_CHOICES[] = {4, 8, 16, 32};
+constexpr int PMAX_CHOICES[] = {4, 8};
+
+}  // namespace sc

@@ -1,0 +1,110 @@
+// Shared host/device data structures of the SumCheck library.
+//
+// HBM layout of a proof (element-major, like the reference's default layout,
+// cuda/src/field_sumcheck.cu:143-163): table v of a round with n entries per table occupies
+// words [v*n*L, (v+1)*n*L), entry i at words [i*L, (i+1)*L), little-endian limbs, Montgomery
+// form. Round j of a proof with num_vars variables has n_j = 2^(num_vars - j) entries per
+// table. Round 0 reads the input tables; round j >= 1 reads round j-1's tables, folds them
+// with r_{j-1} and writes round j's tables into one of two scratch buffers (n/2 and n/4
+// entries per table, used alternately), so a proof needs 1.75x its input in HBM.
+#pragma once
+
+#include <stdint.h>
+
+namespace sc {
+
+constexpr int MAX_TABLES = 32;
+constexpr int MAX_TERMS = 32;
+constexpr int MAX_POINTS = 8;  // degree <= 7, as cuda/src/field_sumcheck.cu:29
+constexpr int MAX_ROUNDS = 40;
+constexpr int MAX_LIMBS = 8;
+constexpr int BLOCK = 256;
+constexpr int MAX_PASS_BLOCKS = 4096;
+
+enum CoeffKind : int32_t { COEFF_ONE = 0, COEFF_MINUS_ONE = 1, COEFF_GENERAL = 2 };
+enum ChallengeMode : int32_t { CHALLENGE_FIXED = 0, CHALLENGE_SHA3 = 1, CHALLENGE_EXPLICIT = 2 };
+enum FillKind : int32_t { FILL_AFFINE = 0, FILL_UNIFORM = 1 };
+
+// Composition sum_k c_k prod_v x_v^e_kv, read by the kernels with wave-uniform loads.
+struct CompDesc {
+  int32_t num_tables;
+  int32_t num_terms;
+  int32_t npts;  // degree + 1
+  int32_t pad0;
+  uint32_t exps[MAX_TERMS][MAX_TABLES / 8];  // 4-bit exponent of table v in term k
+  int32_t kind[MAX_TERMS];
+  uint32_t coeff[MAX_TERMS][MAX_LIMBS];  // Montgomery form, used when kind == COEFF_GENERAL
+};
+
+struct FieldConsts {
+  uint32_t one[MAX_LIMBS];               // R mod p, the Montgomery image of 1
+  uint32_t r2[MAX_LIMBS];                // R^2 mod p
+  uint32_t word_pow[8][MAX_LIMBS];       // 2^(32i) * R^2 mod p: digest word i -> Montgomery
+};
+
+// Per-proof device state. polys and r are written by the kernel that finishes a round.
+struct ProofState {
+  uint32_t counter;  // blocks of the current pass that have finished
+  uint32_t pad[63];
+  uint32_t r[MAX_ROUNDS][MAX_LIMBS];                     // challenge after round j, Montgomery
+  uint32_t polys[MAX_ROUNDS][MAX_POINTS][MAX_LIMBS];     // g_j(t), Montgomery
+};
+
+struct PassArgs {
+  const uint32_t* src;
+  uint32_t* dst;
+  uint64_t src_n;  // entries per table in src
+  uint64_t pairs;  // pairs of round `round`
+  int32_t round;
+  int32_t num_rounds;
+  int32_t fold;  // 0 for round 0 (pairs read straight from src), 1 otherwise
+  int32_t challenge_mode;
+  const CompDesc* comp;
+  const FieldConsts* fc;
+  ProofState* st;
+  uint32_t* partials;         // [gridDim][MAX_POINTS][L]
+  const uint32_t* explicit_r; // [num_rounds-1][L] canonical, CHALLENGE_EXPLICIT only
+};
+
+// Rounds first_round..num_rounds-1 in one workgroup. Round j >= 1 reads buf(j-1) and writes
+// buf(j), where buf(0) = input and buf(k) = (k odd ? buf_a : buf_b); n_j = n0 >> j.
+struct TailArgs {
+  const uint32_t* input;
+  uint32_t* buf_a;
+  uint32_t* buf_b;
+  uint64_t n0;  // entries per table of the input (2^num_vars)
+  int32_t first_round;
+  int32_t num_rounds;
+  int32_t challenge_mode;
+  int32_t pad0;
+  const CompDesc* comp;
+  const FieldConsts* fc;
+  ProofState* st;
+  const uint32_t* explicit_r;
+};
+
+struct FinalizeArgs {
+  const uint32_t* partials;  // [nblocks][MAX_POINTS][L]
+  uint32_t nblocks;
+  int32_t round;
+  int32_t num_rounds;
+  int32_t challenge_mode;
+  const CompDesc* comp;
+  const FieldConsts* fc;
+  ProofState* st;
+  const uint32_t* explicit_r;
+};
+
+struct FillArgs {
+  uint32_t* tables;
+  uint64_t n;  // entries per table
+  int32_t num_tables;
+  int32_t kind;
+  uint64_t seed;
+  uint64_t selector_mask;
+  uint32_t instance;
+  uint32_t pad0;
+  const FieldConsts* fc;
+};
+
+}  // namespace sc

@@ -1,0 +1,699 @@
+// Host runtime and C ABI of libsumcheck_hip.so (declared in include/sumcheck_hip.h).
+//
+// A context owns one HIP stream and the per-field constant blocks; a proof owns its device
+// buffers (input tables unless borrowed, two fold scratch buffers, block partials, the
+// composition descriptor and the round state). A run enqueues the whole round loop without
+// host synchronization: the reference's run_once (cuda/src/field_sumcheck.cu:1012-1088)
+// with the per-point D2H copies and host hashing replaced by device finalize kernels.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sumcheck_hip.h"
+#include "field.hpp"
+#include "launch.hpp"
+#include "layout.hpp"
+
+using namespace sc;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+#define SC_HIP(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t _e = (expr);                                                                   \
+    if (_e != hipSuccess) {                                                                   \
+      return fail(_e == hipErrorOutOfMemory ? SC_ERR_MEMORY : SC_ERR_HIP,                     \
+                  std::string(#expr) + ": " + hipGetErrorString(_e));                         \
+    }                                                                                         \
+  } while (0)
+
+int field_index(int bit_width) {
+  switch (bit_width) {
+    case 32: return 0;
+    case 64: return 1;
+    case 128: return 2;
+    case 256: return 3;
+    default: return -1;
+  }
+}
+
+const Launchers& launchers_for(int idx) {
+  switch (idx) {
+    case 0: return launchers_p32();
+    case 1: return launchers_p64();
+    case 2: return launchers_p128();
+    default: return launchers_p256();
+  }
+}
+
+// Host-side constants and conversions, using the same templates as the device.
+template <class F>
+struct HostField {
+  static constexpr int L = F::L;
+  static Elem<L> from(const uint32_t* w) {
+    Elem<L> x;
+    for (int i = 0; i < L; ++i) x.v[i] = w[i];
+    return x;
+  }
+  static bool less_than_p(const Elem<L>& x) {
+    Elem<L> t;
+    return sub_p<F>(t, x) != 0u;
+  }
+  static Elem<L> dbl_times(Elem<L> x, int k) {
+    for (int i = 0; i < k; ++i) x = add<F>(x, x);
+    return x;
+  }
+  static void consts(FieldConsts& fc) {
+    std::memset(&fc, 0, sizeof(fc));
+    Elem<L> one = zero<F>();
+    one.v[0] = 1u;
+    const Elem<L> rmod = dbl_times(one, 32 * L);   // R mod p
+    const Elem<L> r2 = dbl_times(rmod, 32 * L);    // R^2 mod p
+    for (int i = 0; i < L; ++i) {
+      fc.one[i] = rmod.v[i];
+      fc.r2[i] = r2.v[i];
+    }
+    for (int w = 0; w < 8; ++w) {
+      const Elem<L> pw = dbl_times(r2, 32 * w);
+      for (int i = 0; i < L; ++i) fc.word_pow[w][i] = pw.v[i];
+    }
+  }
+  static Elem<L> to_mont(const Elem<L>& x, const FieldConsts& fc) {
+    return mont_mul<F>(x, from(fc.r2));
+  }
+  static bool is_minus_one(const Elem<L>& x) {
+    Elem<L> one = zero<F>();
+    one.v[0] = 1u;
+    const Elem<L> s = add<F>(x, one);
+    return is_zero<F>(s);
+  }
+};
+
+template <class F>
+int build_comp(const sc_composition* c, const FieldConsts& fc, CompDesc& d, int& degree) {
+  using H = HostField<F>;
+  constexpr int L = F::L;
+  if (c == nullptr || c->term_lens == nullptr || c->term_offsets == nullptr || c->term_vars == nullptr)
+    return fail(SC_ERR_ARG, "composition pointers must not be NULL");
+  if (c->num_tables < 1 || c->num_tables > MAX_TABLES)
+    return fail(SC_ERR_UNSUPPORTED, "num_tables must be in [1, 32]");
+  if (c->num_terms < 1 || c->num_terms > MAX_TERMS)
+    return fail(SC_ERR_UNSUPPORTED, "num_terms must be in [1, 32]");
+  std::memset(&d, 0, sizeof(d));
+  d.num_tables = c->num_tables;
+  d.num_terms = c->num_terms;
+  degree = 0;
+  for (int k = 0; k < c->num_terms; ++k) {
+    const int len = c->term_lens[k];
+    if (len < 1) return fail(SC_ERR_ARG, "every term needs at least one factor");
+    if (len > MAX_POINTS - 1) return fail(SC_ERR_UNSUPPORTED, "term degree must be <= 7");
+    if (len > degree) degree = len;
+    int counts[MAX_TABLES] = {0};
+    for (int m = 0; m < len; ++m) {
+      const int v = c->term_vars[c->term_offsets[k] + m];
+      if (v < 0 || v >= c->num_tables) return fail(SC_ERR_ARG, "term_vars entry out of range");
+      ++counts[v];
+    }
+    for (int v = 0; v < MAX_TABLES; ++v) d.exps[k][v >> 3] |= (uint32_t)counts[v] << ((v & 7) * 4);
+    Elem<L> coeff = zero<F>();
+    if (c->coeffs != nullptr) coeff = H::from(c->coeffs + (size_t)k * L);
+    else coeff.v[0] = 1u;
+    if (!H::less_than_p(coeff)) return fail(SC_ERR_ARG, "coefficient must be canonical (< p)");
+    if (coeff.v[0] == 1u && [&] { for (int i = 1; i < L; ++i) if (coeff.v[i]) return false; return true; }()) {
+      d.kind[k] = COEFF_ONE;
+    } else if (H::is_minus_one(coeff)) {
+      d.kind[k] = COEFF_MINUS_ONE;
+    } else {
+      d.kind[k] = COEFF_GENERAL;
+    }
+    const Elem<L> cm = H::to_mont(coeff, fc);
+    for (int i = 0; i < L; ++i) d.coeff[k][i] = cm.v[i];
+  }
+  d.npts = degree + 1;
+  return SC_OK;
+}
+
+int build_comp_any(int fidx, const sc_composition* c, const FieldConsts& fc, CompDesc& d, int& degree) {
+  switch (fidx) {
+    case 0: return build_comp<FieldP32>(c, fc, d, degree);
+    case 1: return build_comp<FieldP64>(c, fc, d, degree);
+    case 2: return build_comp<FieldP128>(c, fc, d, degree);
+    default: return build_comp<FieldP256>(c, fc, d, degree);
+  }
+}
+
+void consts_any(int fidx, FieldConsts& fc) {
+  switch (fidx) {
+    case 0: HostField<FieldP32>::consts(fc); break;
+    case 1: HostField<FieldP64>::consts(fc); break;
+    case 2: HostField<FieldP128>::consts(fc); break;
+    default: HostField<FieldP256>::consts(fc); break;
+  }
+}
+
+int pick_tmax(int T) {
+  for (int t : TMAX_CHOICES)
+    if (T <= t) return t;
+  return -1;
+}
+
+}  // namespace
+
+struct sc_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  FieldConsts host_fc[4];
+  FieldConsts* d_fc[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+struct sc_proof {
+  sc_context* ctx = nullptr;
+  int fidx = 0, L = 0, T = 0, npts = 0, num_vars = 0, tmax = 0, pmax = 0;
+  const Launchers* ln = nullptr;
+  uint64_t n0 = 0;
+  uint32_t* d_tables = nullptr;
+  bool owns_tables = false;
+  uint32_t* d_a = nullptr;
+  uint32_t* d_b = nullptr;
+  uint32_t* d_partials = nullptr;
+  uint32_t* d_explicit = nullptr;
+  CompDesc* d_comp = nullptr;
+  ProofState* d_state = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int max_pass_blocks = 2048;
+  int64_t tail_pairs = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_mode = -1;
+  uint64_t device_bytes = 0;
+  bool profiling = false;
+  bool profiled_last = false;
+  hipEvent_t prof_ev[2 * (MAX_ROUNDS + 1)] = {};
+};
+
+namespace {
+
+int64_t default_tail_pairs(int L) {
+  // One workgroup handles the last rounds; keep its work small next to a pass launch.
+  switch (L) {
+    case 1: return 4096;
+    case 2: return 2048;
+    case 4: return 512;
+    default: return 128;
+  }
+}
+
+// First round that runs in the tail kernel.
+int tail_first_round(const sc_proof* p) {
+  int j = 0;
+  while (j < p->num_vars && (int64_t)(p->n0 >> (j + 1)) > p->tail_pairs) ++j;
+  return j;
+}
+
+uint32_t* buf_for_round(const sc_proof* p, int j) {
+  if (j == 0) return p->d_tables;
+  return (j & 1) ? p->d_a : p->d_b;
+}
+
+int enqueue_rounds(sc_proof* p, int mode) {
+  hipStream_t s = p->ctx->stream;
+  const FieldConsts* fc = p->ctx->d_fc[p->fidx];
+  const int jt = tail_first_round(p);
+  for (int j = 0; j < jt; ++j) {
+    PassArgs a;
+    a.src = buf_for_round(p, j == 0 ? 0 : j - 1);
+    a.dst = j == 0 ? nullptr : buf_for_round(p, j);
+    a.src_n = j == 0 ? p->n0 : (p->n0 >> (j - 1));
+    a.pairs = p->n0 >> (j + 1);
+    a.round = j;
+    a.num_rounds = p->num_vars;
+    a.fold = j > 0;
+    a.challenge_mode = mode;
+    a.comp = p->d_comp;
+    a.fc = fc;
+    a.st = p->d_state;
+    a.partials = p->d_partials;
+    a.explicit_r = p->d_explicit;
+    uint64_t g = (a.pairs + BLOCK - 1) / BLOCK;
+    if (g > (uint64_t)p->max_pass_blocks) g = (uint64_t)p->max_pass_blocks;
+    if (p->profiling) SC_HIP(hipEventRecord(p->prof_ev[2 * j], s));
+    SC_HIP(p->ln->pass(s, p->tmax, p->pmax, a, (unsigned)g));
+    if (p->profiling) SC_HIP(hipEventRecord(p->prof_ev[2 * j + 1], s));
+    FinalizeArgs f;
+    f.partials = p->d_partials;
+    f.nblocks = (uint32_t)g;
+    f.round = j;
+    f.num_rounds = p->num_vars;
+    f.challenge_mode = mode;
+    f.comp = p->d_comp;
+    f.fc = fc;
+    f.st = p->d_state;
+    f.explicit_r = p->d_explicit;
+    SC_HIP(p->ln->finalize(s, p->pmax, f));
+  }
+  if (jt < p->num_vars) {
+    TailArgs t;
+    t.input = p->d_tables;
+    t.buf_a = p->d_a;
+    t.buf_b = p->d_b;
+    t.n0 = p->n0;
+    t.first_round = jt;
+    t.num_rounds = p->num_vars;
+    t.challenge_mode = mode;
+    t.pad0 = 0;
+    t.comp = p->d_comp;
+    t.fc = fc;
+    t.st = p->d_state;
+    t.explicit_r = p->d_explicit;
+    if (p->profiling) SC_HIP(hipEventRecord(p->prof_ev[2 * MAX_ROUNDS], s));
+    SC_HIP(p->ln->tail(s, p->tmax, p->pmax, t));
+    if (p->profiling) SC_HIP(hipEventRecord(p->prof_ev[2 * MAX_ROUNDS + 1], s));
+  }
+  return SC_OK;
+}
+
+int check_mode(const sc_proof* p, int mode, const uint32_t* explicit_r) {
+  if (mode != SC_CHALLENGE_FIXED && mode != SC_CHALLENGE_SHA3 && mode != SC_CHALLENGE_EXPLICIT)
+    return fail(SC_ERR_ARG, "challenge_mode must be 0 (fixed), 1 (sha3) or 2 (explicit)");
+  if (mode == SC_CHALLENGE_EXPLICIT && p->num_vars > 1 && explicit_r == nullptr)
+    return fail(SC_ERR_ARG, "explicit challenges required");
+  return SC_OK;
+}
+
+int upload_explicit(sc_proof* p, int mode, const uint32_t* explicit_r) {
+  if (mode != SC_CHALLENGE_EXPLICIT || p->num_vars < 2) return SC_OK;
+  // Validate canonical range on the host.
+  for (int j = 0; j < p->num_vars - 1; ++j) {
+    const uint32_t* w = explicit_r + (size_t)j * p->L;
+    bool ok;
+    switch (p->fidx) {
+      case 0: ok = HostField<FieldP32>::less_than_p(HostField<FieldP32>::from(w)); break;
+      case 1: ok = HostField<FieldP64>::less_than_p(HostField<FieldP64>::from(w)); break;
+      case 2: ok = HostField<FieldP128>::less_than_p(HostField<FieldP128>::from(w)); break;
+      default: ok = HostField<FieldP256>::less_than_p(HostField<FieldP256>::from(w)); break;
+    }
+    if (!ok) return fail(SC_ERR_ARG, "explicit challenge must be canonical (< p)");
+  }
+  SC_HIP(hipMemcpyAsync(p->d_explicit, explicit_r, sizeof(uint32_t) * p->L * (p->num_vars - 1),
+                        hipMemcpyHostToDevice, p->ctx->stream));
+  SC_HIP(hipStreamSynchronize(p->ctx->stream));
+  return SC_OK;
+}
+
+void free_proof(sc_proof* p) {
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->owns_tables && p->d_tables) (void)hipFree(p->d_tables);
+  if (p->d_a) (void)hipFree(p->d_a);
+  if (p->d_b) (void)hipFree(p->d_b);
+  if (p->d_partials) (void)hipFree(p->d_partials);
+  if (p->d_explicit) (void)hipFree(p->d_explicit);
+  if (p->d_comp) (void)hipFree(p->d_comp);
+  if (p->d_state) (void)hipFree(p->d_state);
+  if (p->ev0) (void)hipEventDestroy(p->ev0);
+  if (p->ev1) (void)hipEventDestroy(p->ev1);
+  for (hipEvent_t e : p->prof_ev)
+    if (e) (void)hipEventDestroy(e);
+  delete p;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sc_last_error(void) { return g_error.c_str(); }
+
+const char* sc_version(void) { return "sumcheck_hip 0.1 gfx950"; }
+
+int sc_context_create(int device, sc_context** out) {
+  if (out == nullptr) return fail(SC_ERR_ARG, "out must not be NULL");
+  *out = nullptr;
+  int count = 0;
+  SC_HIP(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) return fail(SC_ERR_ARG, "no such device");
+  SC_HIP(hipSetDevice(device));
+  sc_context* c = new (std::nothrow) sc_context();
+  if (c == nullptr) return fail(SC_ERR_MEMORY, "host allocation failed");
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(SC_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  for (int f = 0; f < 4; ++f) {
+    consts_any(f, c->host_fc[f]);
+    e = hipMalloc(&c->d_fc[f], sizeof(FieldConsts));
+    if (e == hipSuccess) e = hipMemcpy(c->d_fc[f], &c->host_fc[f], sizeof(FieldConsts), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      sc_context_destroy(c);
+      return fail(SC_ERR_HIP, std::string("field constants: ") + hipGetErrorString(e));
+    }
+  }
+  *out = c;
+  return SC_OK;
+}
+
+int sc_context_destroy(sc_context* ctx) {
+  if (ctx == nullptr) return SC_OK;
+  (void)hipSetDevice(ctx->device);
+  for (int f = 0; f < 4; ++f)
+    if (ctx->d_fc[f]) (void)hipFree(ctx->d_fc[f]);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return SC_OK;
+}
+
+int sc_context_sync(sc_context* ctx) {
+  if (ctx == nullptr) return fail(SC_ERR_ARG, "ctx must not be NULL");
+  SC_HIP(hipStreamSynchronize(ctx->stream));
+  return SC_OK;
+}
+
+int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* comp, int32_t num_vars,
+                    const sc_input* input, sc_proof** out) {
+  if (ctx == nullptr || out == nullptr || input == nullptr)
+    return fail(SC_ERR_ARG, "ctx, input and out must not be NULL");
+  *out = nullptr;
+  const int fidx = field_index(bit_width);
+  if (fidx < 0) return fail(SC_ERR_ARG, "bit_width must be 32, 64, 128 or 256");
+  if (num_vars < 1 || num_vars > MAX_ROUNDS) return fail(SC_ERR_UNSUPPORTED, "num_vars must be in [1, 40]");
+  CompDesc desc;
+  int degree = 0;
+  int rc = build_comp_any(fidx, comp, ctx->host_fc[fidx], desc, degree);
+  if (rc != SC_OK) return rc;
+  if (input->kind < SC_INPUT_HOST_CANONICAL || input->kind > SC_INPUT_UNIFORM)
+    return fail(SC_ERR_ARG, "unknown input kind");
+  if (input->kind <= SC_INPUT_DEVICE_MONT && input->data == nullptr)
+    return fail(SC_ERR_ARG, "input data must not be NULL");
+
+  SC_HIP(hipSetDevice(ctx->device));
+  sc_proof* p = new (std::nothrow) sc_proof();
+  if (p == nullptr) return fail(SC_ERR_MEMORY, "host allocation failed");
+  p->ctx = ctx;
+  p->fidx = fidx;
+  p->ln = &launchers_for(fidx);
+  p->L = p->ln->limbs;
+  p->T = desc.num_tables;
+  p->npts = desc.npts;
+  p->num_vars = num_vars;
+  p->tmax = pick_tmax(p->T);
+  p->pmax = p->npts <= 4 ? 4 : 8;
+  p->n0 = (uint64_t)1 << num_vars;
+  p->tail_pairs = default_tail_pairs(p->L);
+
+  const uint64_t words = (uint64_t)p->T * p->n0 * p->L;
+  auto cleanup = [&](int code) {
+    free_proof(p);
+    return code;
+  };
+  auto alloc = [&](void** ptr, uint64_t bytes) -> hipError_t {
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 16);
+    if (e == hipSuccess) p->device_bytes += bytes;
+    return e;
+  };
+#define SC_HIP_P(expr)                                                                        \
+  do {                                                                                        \
+    hipError_t _e = (expr);                                                                   \
+    if (_e != hipSuccess) {                                                                   \
+      fail(_e == hipErrorOutOfMemory ? SC_ERR_MEMORY : SC_ERR_HIP,                            \
+           std::string(#expr) + ": " + hipGetErrorString(_e));                                \
+      return cleanup(_e == hipErrorOutOfMemory ? SC_ERR_MEMORY : SC_ERR_HIP);                 \
+    }                                                                                         \
+  } while (0)
+
+  if (input->kind == SC_INPUT_DEVICE_MONT) {
+    p->d_tables = (uint32_t*)input->data;
+    p->owns_tables = false;
+  } else {
+    SC_HIP_P(alloc((void**)&p->d_tables, words * 4));
+    p->owns_tables = true;
+  }
+  SC_HIP_P(alloc((void**)&p->d_a, (words / 2) * 4));
+  SC_HIP_P(alloc((void**)&p->d_b, (words / 4) * 4));
+  SC_HIP_P(alloc((void**)&p->d_partials, (uint64_t)MAX_PASS_BLOCKS * MAX_POINTS * p->L * 4));
+  SC_HIP_P(alloc((void**)&p->d_explicit, (uint64_t)MAX_ROUNDS * MAX_LIMBS * 4));
+  SC_HIP_P(alloc((void**)&p->d_comp, sizeof(CompDesc)));
+  SC_HIP_P(alloc((void**)&p->d_state, sizeof(ProofState)));
+  SC_HIP_P(hipEventCreate(&p->ev0));
+  SC_HIP_P(hipEventCreate(&p->ev1));
+  hipStream_t s = ctx->stream;
+  SC_HIP_P(hipMemcpyAsync(p->d_comp, &desc, sizeof(desc), hipMemcpyHostToDevice, s));
+  SC_HIP_P(hipMemsetAsync(p->d_state, 0, sizeof(ProofState), s));
+
+  const FieldConsts* fc = ctx->d_fc[fidx];
+  if (input->kind == SC_INPUT_HOST_CANONICAL || input->kind == SC_INPUT_HOST_MONT) {
+    SC_HIP_P(hipMemcpyAsync(p->d_tables, input->data, words * 4, hipMemcpyHostToDevice, s));
+    if (input->kind == SC_INPUT_HOST_CANONICAL) {
+      SC_HIP_P(p->ln->field_op(s, 3, (uint64_t)p->T * p->n0, p->d_tables, nullptr, p->d_tables, fc));
+    }
+  } else if (input->kind == SC_INPUT_AFFINE || input->kind == SC_INPUT_UNIFORM) {
+    FillArgs f;
+    f.tables = p->d_tables;
+    f.n = p->n0;
+    f.num_tables = p->T;
+    f.kind = input->kind == SC_INPUT_AFFINE ? FILL_AFFINE : FILL_UNIFORM;
+    f.seed = input->seed;
+    f.selector_mask = input->selector_mask;
+    f.instance = input->instance;
+    f.pad0 = 0;
+    f.fc = fc;
+    SC_HIP_P(p->ln->fill(s, f));
+  }
+  SC_HIP_P(hipStreamSynchronize(s));
+#undef SC_HIP_P
+  *out = p;
+  return SC_OK;
+}
+
+int sc_proof_set_schedule(sc_proof* p, int32_t max_pass_blocks, int64_t tail_pairs) {
+  if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
+  if (max_pass_blocks < 0 || max_pass_blocks > MAX_PASS_BLOCKS)
+    return fail(SC_ERR_ARG, "max_pass_blocks must be in [0, 4096]");
+  if (tail_pairs < 0) return fail(SC_ERR_ARG, "tail_pairs must be >= 0");
+  p->max_pass_blocks = max_pass_blocks ? max_pass_blocks : 2048;
+  p->tail_pairs = tail_pairs ? tail_pairs : default_tail_pairs(p->L);
+  if (p->graph_exec) {
+    (void)hipGraphExecDestroy(p->graph_exec);
+    (void)hipGraphDestroy(p->graph);
+    p->graph_exec = nullptr;
+    p->graph = nullptr;
+    p->graph_mode = -1;
+  }
+  return SC_OK;
+}
+
+int sc_proof_run(sc_proof* p, int32_t mode, const uint32_t* explicit_r) {
+  if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
+  int rc = check_mode(p, mode, explicit_r);
+  if (rc != SC_OK) return rc;
+  SC_HIP(hipSetDevice(p->ctx->device));
+  rc = upload_explicit(p, mode, explicit_r);
+  if (rc != SC_OK) return rc;
+  hipStream_t s = p->ctx->stream;
+  SC_HIP(hipEventRecord(p->ev0, s));
+  if (p->graph_exec != nullptr && p->graph_mode == mode && !p->profiling) {
+    SC_HIP(hipGraphLaunch(p->graph_exec, s));
+    p->profiled_last = false;
+  } else {
+    rc = enqueue_rounds(p, mode);
+    if (rc != SC_OK) return rc;
+    p->profiled_last = p->profiling;
+  }
+  SC_HIP(hipEventRecord(p->ev1, s));
+  return SC_OK;
+}
+
+int sc_proof_capture(sc_proof* p, int32_t mode, const uint32_t* explicit_r) {
+  if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
+  int rc = check_mode(p, mode, explicit_r);
+  if (rc != SC_OK) return rc;
+  SC_HIP(hipSetDevice(p->ctx->device));
+  rc = upload_explicit(p, mode, explicit_r);
+  if (rc != SC_OK) return rc;
+  if (p->graph_exec) {
+    (void)hipGraphExecDestroy(p->graph_exec);
+    (void)hipGraphDestroy(p->graph);
+    p->graph_exec = nullptr;
+    p->graph = nullptr;
+  }
+  hipStream_t s = p->ctx->stream;
+  const bool prof = p->profiling;
+  p->profiling = false;  // no event records inside the graph
+  SC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  rc = enqueue_rounds(p, mode);
+  p->profiling = prof;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc != SC_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(SC_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  p->graph = g;
+  SC_HIP(hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0));
+  p->graph_mode = mode;
+  return SC_OK;
+}
+
+int sc_proof_wait(sc_proof* p, float* ms) {
+  if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
+  SC_HIP(hipSetDevice(p->ctx->device));
+  SC_HIP(hipStreamSynchronize(p->ctx->stream));
+  if (ms != nullptr) {
+    *ms = 0.0f;
+    SC_HIP(hipEventElapsedTime(ms, p->ev0, p->ev1));
+  }
+  return SC_OK;
+}
+
+int sc_proof_result(sc_proof* p, uint32_t* round_polys, uint32_t* challenges, uint64_t* checksum) {
+  if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
+  SC_HIP(hipSetDevice(p->ctx->device));
+  SC_HIP(hipStreamSynchronize(p->ctx->stream));
+  std::vector<ProofState> host(1);
+  SC_HIP(hipMemcpy(host.data(), p->d_state, sizeof(ProofState), hipMemcpyDeviceToHost));
+  const ProofState& st = host[0];
+  uint64_t sum = 0;
+  for (int j = 0; j < p->num_vars; ++j) {
+    for (int t = 0; t < p->npts; ++t) {
+      for (int i = 0; i < p->L; ++i) {
+        const uint32_t w = st.polys[j][t][i];
+        sum += (uint64_t)(i + 1) * w;
+        if (round_polys) round_polys[((size_t)j * p->npts + t) * p->L + i] = w;
+      }
+    }
+    if (challenges && j + 1 < p->num_vars) {
+      for (int i = 0; i < p->L; ++i) challenges[(size_t)j * p->L + i] = st.r[j][i];
+    }
+  }
+  if (checksum) *checksum = sum;
+  return SC_OK;
+}
+
+int sc_proof_tables(sc_proof* p, uint32_t* out) {
+  if (p == nullptr || out == nullptr) return fail(SC_ERR_ARG, "proof and out must not be NULL");
+  SC_HIP(hipSetDevice(p->ctx->device));
+  SC_HIP(hipStreamSynchronize(p->ctx->stream));
+  SC_HIP(hipMemcpy(out, p->d_tables, (uint64_t)p->T * p->n0 * p->L * 4, hipMemcpyDeviceToHost));
+  return SC_OK;
+}
+
+int sc_proof_info(sc_proof* p, int32_t* degree, int32_t* pass_rounds, uint64_t* device_bytes) {
+  if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
+  if (degree) *degree = p->npts - 1;
+  if (pass_rounds) *pass_rounds = tail_first_round(p);
+  if (device_bytes) *device_bytes = p->device_bytes;
+  return SC_OK;
+}
+
+int sc_proof_set_profiling(sc_proof* p, int32_t enable) {
+  if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
+  SC_HIP(hipSetDevice(p->ctx->device));
+  if (enable && p->prof_ev[0] == nullptr) {
+    for (hipEvent_t& e : p->prof_ev) SC_HIP(hipEventCreate(&e));
+  }
+  p->profiling = enable != 0;
+  return SC_OK;
+}
+
+int sc_proof_kernel_times(sc_proof* p, float* pass_ms, int32_t capacity, float* tail_ms) {
+  if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
+  if (!p->profiled_last) return fail(SC_ERR_ARG, "the last run was not profiled (sc_proof_set_profiling)");
+  SC_HIP(hipSetDevice(p->ctx->device));
+  SC_HIP(hipStreamSynchronize(p->ctx->stream));
+  const int jt = tail_first_round(p);
+  for (int j = 0; j < jt && j < capacity; ++j) {
+    float ms = 0.0f;
+    SC_HIP(hipEventElapsedTime(&ms, p->prof_ev[2 * j], p->prof_ev[2 * j + 1]));
+    if (pass_ms) pass_ms[j] = ms;
+  }
+  if (tail_ms) {
+    *tail_ms = 0.0f;
+    if (jt < p->num_vars) SC_HIP(hipEventElapsedTime(tail_ms, p->prof_ev[2 * MAX_ROUNDS], p->prof_ev[2 * MAX_ROUNDS + 1]));
+  }
+  return SC_OK;
+}
+
+int sc_proof_destroy(sc_proof* p) {
+  if (p == nullptr) return SC_OK;
+  (void)hipSetDevice(p->ctx->device);
+  (void)hipStreamSynchronize(p->ctx->stream);
+  free_proof(p);
+  return SC_OK;
+}
+
+int sc_prove(sc_context* ctx, int32_t bit_width, const sc_composition* comp, int32_t num_vars,
+             const sc_input* input, int32_t mode, const uint32_t* explicit_r, uint32_t* round_polys,
+             uint32_t* challenges, uint64_t* checksum) {
+  sc_proof* p = nullptr;
+  int rc = sc_proof_create(ctx, bit_width, comp, num_vars, input, &p);
+  if (rc != SC_OK) return rc;
+  rc = sc_proof_run(p, mode, explicit_r);
+  if (rc == SC_OK) rc = sc_proof_wait(p, nullptr);
+  if (rc == SC_OK) rc = sc_proof_result(p, round_polys, challenges, checksum);
+  std::string err = g_error;
+  sc_proof_destroy(p);
+  g_error = err;
+  return rc;
+}
+
+int sc_field_op(sc_context* ctx, int32_t bit_width, int32_t op, uint64_t n, const uint32_t* a,
+                const uint32_t* b, uint32_t* out) {
+  if (ctx == nullptr || a == nullptr || out == nullptr) return fail(SC_ERR_ARG, "NULL argument");
+  const int fidx = field_index(bit_width);
+  if (fidx < 0) return fail(SC_ERR_ARG, "bit_width must be 32, 64, 128 or 256");
+  if (op < 0 || op > 4) return fail(SC_ERR_ARG, "op must be in [0, 4]");
+  if (op <= 2 && b == nullptr) return fail(SC_ERR_ARG, "b must not be NULL for this op");
+  if (n == 0) return SC_OK;
+  SC_HIP(hipSetDevice(ctx->device));
+  const Launchers& ln = launchers_for(fidx);
+  const uint64_t bytes = n * ln.limbs * 4;
+  uint32_t* d = nullptr;
+  SC_HIP(hipMalloc(&d, 3 * bytes));
+  hipStream_t s = ctx->stream;
+  hipError_t e = hipMemcpyAsync(d, a, bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && b != nullptr) e = hipMemcpyAsync(d + n * ln.limbs, b, bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = ln.field_op(s, op, n, d, d + n * ln.limbs, d + 2 * n * ln.limbs, ctx->d_fc[fidx]);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d + 2 * n * ln.limbs, bytes, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(SC_ERR_HIP, std::string("sc_field_op: ") + hipGetErrorString(e));
+  return SC_OK;
+}
+
+int sc_transcript_challenge(sc_context* ctx, int32_t bit_width, int32_t round, const uint32_t* g_mont,
+                            int32_t npts, uint32_t* r_mont) {
+  if (ctx == nullptr || g_mont == nullptr || r_mont == nullptr) return fail(SC_ERR_ARG, "NULL argument");
+  const int fidx = field_index(bit_width);
+  if (fidx < 0) return fail(SC_ERR_ARG, "bit_width must be 32, 64, 128 or 256");
+  if (npts < 1 || npts > MAX_POINTS) return fail(SC_ERR_ARG, "npts must be in [1, 8]");
+  if (round < 0 || round + 1 >= MAX_ROUNDS) return fail(SC_ERR_ARG, "round out of range");
+  SC_HIP(hipSetDevice(ctx->device));
+  const Launchers& ln = launchers_for(fidx);
+  uint32_t* d_g = nullptr;
+  ProofState* d_st = nullptr;
+  SC_HIP(hipMalloc(&d_g, (size_t)npts * ln.limbs * 4));
+  hipError_t e = hipMalloc(&d_st, sizeof(ProofState));
+  hipStream_t s = ctx->stream;
+  if (e == hipSuccess) e = hipMemcpyAsync(d_g, g_mont, (size_t)npts * ln.limbs * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = ln.transcript(s, d_g, npts, round, ctx->d_fc[fidx], d_st);
+  if (e == hipSuccess) e = hipMemcpyAsync(r_mont, d_st->r[round], (size_t)ln.limbs * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(d_g);
+  if (d_st) (void)hipFree(d_st);
+  if (e != hipSuccess) return fail(SC_ERR_HIP, std::string("sc_transcript_challenge: ") + hipGetErrorString(e));
+  return SC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+"""Field-sweep driver: the reference's ``_time_one`` contract on the HIP library.
+
+``time_one(workload, bit_width, rounds, warmups, repeats, seed, timing_mode, challenge_mode,
+layout, point_mode)`` prepares one case, runs it ``1 + warmups + repeats`` times and returns
+the reference's CSV row (benchmarks/field_sweep/triton_sweep.py:626-816; columns of
+cuda/src/field_sumcheck.cu:1336), with the checksum summed over all runs as the reference
+does (field_sumcheck.cu:1222-1285), plus ``round_polys`` of the last run and throughput
+columns.
+
+Reference-compatible inputs are the affine tables with seed ``seed + rounds``
+(field_sumcheck.cu:1217-1219, triton_sweep.py:642).
+"""
+
+from __future__ import annotations
+
+import statistics
+import time
+
+import numpy as np
+
+from .fields import get_field
+from .prover import Composition, Context, Proof, default_context
+from .workloads import get_spec
+
+CSV_COLUMNS = [
+    "workload", "polynomial", "backend", "field", "arithmetic", "layout", "point_mode",
+    "bit_width", "limbs", "modulus_hex", "rounds", "N", "vars", "degree", "terms", "warmups",
+    "repeats", "timing_mode", "challenge_mode", "first_ms", "median_ms", "eval_median_ms",
+    "challenge_median_ms", "fold_median_ms", "total_median_ms", "input_mib", "checksum", "status",
+]
+EXTRA_COLUMNS = ["proofs_per_s", "p50_ms", "p99_ms", "algo_bytes", "hbm_gbps"]
+
+
+def algorithmic_bytes(num_tables: int, limbs: int, num_vars: int) -> int:
+    """HBM bytes of one fused proof: each round streams its input once and, from round 1 on,
+    writes the folded tables: T*W*(3*2^nv - 4) read + T*W*(2^nv - 2) written, W = 4L."""
+    w = 4 * limbs
+    n = 1 << num_vars
+    return num_tables * w * (3 * n - 4) + num_tables * w * (n - 2)
+
+
+def run_timed(proof: Proof, challenge_mode: str, timer: str = "host", challenges=None) -> float:
+    """One whole proof, synchronized on both sides; ms by host clock or device events."""
+    proof.ctx.sync()
+    t0 = time.perf_counter()
+    proof.run(challenge_mode, challenges)
+    dev_ms = proof.wait()
+    host_ms = (time.perf_counter() - t0) * 1000.0
+    return dev_ms if timer == "events" else host_ms
+
+
+def time_one(
+    workload: str,
+    bit_width: int,
+    rounds: int,
+    warmups: int = 1,
+    repeats: int = 5,
+    seed: int = 1,
+    timing_mode: str = "whole-run",
+    challenge_mode: str = "fixed",
+    layout: str = "element",
+    point_mode: str = "specialized",
+    *,
+    input_mode: str = "affine",
+    timer: str = "host",
+    ctx: Context | None = None,
+) -> dict:
+    if timing_mode not in ("whole-run", "round-sum"):
+        raise ValueError("timing_mode must be whole-run or round-sum")
+    if challenge_mode not in ("fixed", "sha3"):
+        raise ValueError("challenge_mode must be fixed or sha3")
+    if layout != "element":
+        raise ValueError("only the element layout is implemented")
+    if point_mode not in ("generic", "specialized"):
+        raise ValueError("point_mode must be generic or specialized")
+    field = get_field(bit_width)
+    spec = get_spec(workload)
+    comp = Composition.from_spec(spec)
+    ctx = ctx or default_context()
+    n = 1 << rounds
+    in_seed = seed + rounds if input_mode == "affine" else seed
+    with Proof(ctx, bit_width, comp, rounds, input=input_mode, seed=in_seed) as proof:
+        first_ms = run_timed(proof, challenge_mode, timer)
+        _p, _c, checksum = proof.result()
+        for _ in range(warmups):
+            run_timed(proof, challenge_mode, timer)
+            checksum += proof.result()[2]
+        samples = []
+        for _ in range(repeats):
+            samples.append(run_timed(proof, challenge_mode, timer))
+            checksum += proof.result()[2]
+        polys, chal, _cs = proof.result()
+    median_ms = statistics.median(samples) if samples else first_ms
+    sorted_s = sorted(samples) if samples else [first_ms]
+    p99 = sorted_s[min(len(sorted_s) - 1, int(round(0.99 * (len(sorted_s) - 1))))]
+    algo = algorithmic_bytes(spec.num_vars, field.limbs, rounds)
+    fused = timing_mode == "round-sum"
+    row = {
+        "workload": spec.name,
+        "polynomial": spec.polynomial,
+        "backend": f"hip-sumcheck-prime-field-montgomery-{layout}-layout-{point_mode}-points",
+        "field": field.name,
+        "arithmetic": "montgomery",
+        "layout": layout,
+        "point_mode": point_mode,
+        "bit_width": bit_width,
+        "limbs": field.limbs,
+        "modulus_hex": hex(field.modulus),
+        "rounds": rounds,
+        "N": n,
+        "vars": spec.num_vars,
+        "degree": spec.degree,
+        "terms": spec.terms,
+        "warmups": warmups,
+        "repeats": repeats,
+        "timing_mode": timing_mode,
+        "challenge_mode": challenge_mode,
+        "first_ms": f"{first_ms:.3f}",
+        "median_ms": f"{median_ms:.3f}",
+        # fold, eval and challenge run fused in one pass per round, so round-sum timing
+        # reports the whole round under eval.
+        "eval_median_ms": f"{median_ms:.3f}" if fused else "0.000",
+        "challenge_median_ms": "0.000000",
+        "fold_median_ms": "0.000",
+        "total_median_ms": f"{median_ms:.3f}",
+        "input_mib": f"{spec.num_vars * n * field.limbs * 4 / (1024 ** 2):.6f}",
+        "checksum": checksum,
+        "status": "ok",
+        "proofs_per_s": f"{1000.0 / median_ms:.3f}" if median_ms > 0 else "inf",
+        "p50_ms": f"{median_ms:.3f}",
+        "p99_ms": f"{p99:.3f}",
+        "algo_bytes": algo,
+        "hbm_gbps": f"{algo / (median_ms * 1e6):.1f}" if median_ms > 0 else "inf",
+        "round_polys": polys,
+        "challenges": chal,
+    }
+    return row

@@ -1,0 +1,276 @@
+"""Python host objects over the C ABI: compositions, a device context and prepared proofs.
+
+A ``Proof`` is the reference's per-case state (cuda/src/field_sumcheck.cu:1178-1221): tables
+prepared once on the device, then run any number of times (first run, warmups, repeats),
+each run enqueuing the whole round loop with the transcript computed on the GPU.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib
+from .expr import Expression, encode_expression_with_coeffs, expression_vars
+from .fields import FieldSpec, get_field
+from .workloads import WorkloadSpec
+
+INPUT_KINDS = {
+    "host-canonical": _lib.SC_INPUT_HOST_CANONICAL,
+    "host-mont": _lib.SC_INPUT_HOST_MONT,
+    "device-mont": _lib.SC_INPUT_DEVICE_MONT,
+    "affine": _lib.SC_INPUT_AFFINE,
+    "uniform": _lib.SC_INPUT_UNIFORM,
+}
+
+
+@dataclass(frozen=True)
+class Composition:
+    """sum_k coeffs[k] * prod of tables term_vars[term_offsets[k] : +term_lens[k]].
+
+    Coefficients are integers (negative allowed) or field elements; they are reduced mod p
+    when the composition is bound to a field.
+    """
+
+    num_tables: int
+    coeffs: tuple[int, ...]
+    term_lens: tuple[int, ...]
+    term_offsets: tuple[int, ...]
+    term_vars: tuple[int, ...]
+
+    @property
+    def degree(self) -> int:
+        return max(self.term_lens)
+
+    @property
+    def num_terms(self) -> int:
+        return len(self.term_lens)
+
+    @classmethod
+    def from_spec(cls, spec: WorkloadSpec) -> "Composition":
+        return cls(spec.num_vars, spec.coeffs, spec.term_lens, spec.term_offsets, spec.term_vars)
+
+    @classmethod
+    def from_expression(cls, expression: Expression) -> tuple["Composition", list[str]]:
+        names, coeffs, lens, offs, flat = encode_expression_with_coeffs(expression)
+        return cls(len(names), coeffs, lens, offs, flat), names
+
+    @classmethod
+    def from_terms(cls, num_tables: int, terms: Sequence[tuple[int, Sequence[int]]]) -> "Composition":
+        coeffs, lens, offs, flat = [], [], [], []
+        for c, idx in terms:
+            offs.append(len(flat))
+            coeffs.append(int(c))
+            lens.append(len(idx))
+            flat.extend(int(i) for i in idx)
+        return cls(num_tables, tuple(coeffs), tuple(lens), tuple(offs), tuple(flat))
+
+    def terms(self) -> list[tuple[int, list[int]]]:
+        return [
+            (c, list(self.term_vars[o : o + n]))
+            for c, n, o in zip(self.coeffs, self.term_lens, self.term_offsets)
+        ]
+
+    def bind(self, field: FieldSpec):
+        """The C struct plus the arrays it points into (keep both alive)."""
+        lens = np.asarray(self.term_lens, dtype=np.int32)
+        offs = np.asarray(self.term_offsets, dtype=np.int32)
+        flat = np.asarray(self.term_vars, dtype=np.int32)
+        coeffs = field.ints_to_limbs([c % field.modulus for c in self.coeffs])
+        s = _lib.ScComposition(
+            self.num_tables,
+            self.num_terms,
+            lens.ctypes.data_as(_lib.i32p),
+            offs.ctypes.data_as(_lib.i32p),
+            flat.ctypes.data_as(_lib.i32p),
+            coeffs.ctypes.data_as(_lib.u32p),
+        )
+        return s, (lens, offs, flat, coeffs)
+
+
+class Context:
+    """One device and one HIP stream (sc_context)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        handle = ctypes.c_void_p()
+        _lib.check(self.lib.sc_context_create(int(device), ctypes.byref(handle)))
+        self.handle = handle
+        self.device = int(device)
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.sc_context_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __enter__(self) -> "Context":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def sync(self) -> None:
+        _lib.check(self.lib.sc_context_sync(self.handle))
+
+    def field_op(self, bit_width: int, op: str, a: np.ndarray, b: np.ndarray | None = None) -> np.ndarray:
+        """Elementwise device op on [n][L] limb arrays: mul (Montgomery), add, sub, to_mont, from_mont."""
+        ops = {"mul": 0, "add": 1, "sub": 2, "to_mont": 3, "from_mont": 4}
+        f = get_field(bit_width)
+        a = np.ascontiguousarray(a, dtype=np.uint32).reshape(-1, f.limbs)
+        bb = None if b is None else np.ascontiguousarray(b, dtype=np.uint32).reshape(-1, f.limbs)
+        out = np.empty_like(a)
+        _lib.check(self.lib.sc_field_op(self.handle, f.bit_width, ops[op], a.shape[0],
+                                        _lib.as_u32p(a), _lib.as_u32p(bb), _lib.as_u32p(out)))
+        return out
+
+    def transcript_challenge(self, bit_width: int, round_idx: int, g_mont: np.ndarray) -> np.ndarray:
+        """Device-derived challenge (Montgomery limbs) for g given as [npts][L] Montgomery limbs."""
+        f = get_field(bit_width)
+        g = np.ascontiguousarray(g_mont, dtype=np.uint32).reshape(-1, f.limbs)
+        out = np.empty(f.limbs, dtype=np.uint32)
+        _lib.check(self.lib.sc_transcript_challenge(self.handle, f.bit_width, int(round_idx),
+                                                    _lib.as_u32p(g), g.shape[0], _lib.as_u32p(out)))
+        return out
+
+
+_default_ctx: Context | None = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class Proof:
+    """A prepared SumCheck instance on the device (sc_proof)."""
+
+    def __init__(
+        self,
+        ctx: Context,
+        bit_width: int,
+        comp: Composition,
+        num_vars: int,
+        *,
+        input: str = "uniform",
+        data: np.ndarray | int | None = None,
+        seed: int = 1,
+        instance: int = 0,
+        selector_mask: int = 0,
+    ):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.field = get_field(bit_width)
+        self.comp = comp
+        self.num_vars = int(num_vars)
+        self.npts = comp.degree + 1
+        if input not in INPUT_KINDS:
+            raise ValueError(f"input must be one of {sorted(INPUT_KINDS)}")
+        kind = INPUT_KINDS[input]
+        self._keep = None
+        ptr = None
+        if kind in (_lib.SC_INPUT_HOST_CANONICAL, _lib.SC_INPUT_HOST_MONT):
+            arr = np.ascontiguousarray(data, dtype=np.uint32)
+            expect = comp.num_tables * (1 << self.num_vars) * self.field.limbs
+            if arr.size != expect:
+                raise ValueError(f"input has {arr.size} words, expected {expect}")
+            self._keep = arr
+            ptr = arr.ctypes.data
+        elif kind == _lib.SC_INPUT_DEVICE_MONT:
+            ptr = int(data)
+        inp = _lib.ScInput(kind, int(instance) & 0xFFFFFFFF, ptr, int(seed) & ((1 << 64) - 1),
+                           int(selector_mask) & ((1 << 64) - 1))
+        cs, self._comp_keep = comp.bind(self.field)
+        handle = ctypes.c_void_p()
+        _lib.check(self.lib.sc_proof_create(ctx.handle, self.field.bit_width, ctypes.byref(cs),
+                                            self.num_vars, ctypes.byref(inp), ctypes.byref(handle)))
+        self.handle = handle
+        self._keep = None  # host input has been copied to the device
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.sc_proof_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __enter__(self) -> "Proof":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _explicit(self, mode: int, challenges) -> np.ndarray | None:
+        if mode != _lib.SC_CHALLENGE_EXPLICIT:
+            return None
+        vals = [int(c) % self.field.modulus for c in list(challenges)[: max(self.num_vars - 1, 0)]]
+        if len(vals) < self.num_vars - 1:
+            raise ValueError(f"need {self.num_vars - 1} challenges, got {len(vals)}")
+        if not vals:
+            return np.zeros((1, self.field.limbs), dtype=np.uint32)
+        return self.field.ints_to_limbs(vals)
+
+    def set_schedule(self, max_pass_blocks: int = 0, tail_pairs: int = 0) -> None:
+        _lib.check(self.lib.sc_proof_set_schedule(self.handle, int(max_pass_blocks), int(tail_pairs)))
+
+    def run(self, challenge_mode: str = "sha3", challenges=None) -> None:
+        mode = _lib.CHALLENGE_MODES[challenge_mode]
+        r = self._explicit(mode, challenges)
+        _lib.check(self.lib.sc_proof_run(self.handle, mode, _lib.as_u32p(r)))
+
+    def capture(self, challenge_mode: str = "sha3", challenges=None) -> None:
+        mode = _lib.CHALLENGE_MODES[challenge_mode]
+        r = self._explicit(mode, challenges)
+        _lib.check(self.lib.sc_proof_capture(self.handle, mode, _lib.as_u32p(r)))
+
+    def wait(self) -> float:
+        ms = ctypes.c_float()
+        _lib.check(self.lib.sc_proof_wait(self.handle, ctypes.byref(ms)))
+        return float(ms.value)
+
+    def result(self) -> tuple[np.ndarray, np.ndarray, int]:
+        """(round_polys [nv][d+1][L], challenges [nv-1][L], checksum), Montgomery limbs."""
+        L = self.field.limbs
+        polys = np.zeros((self.num_vars, self.npts, L), dtype=np.uint32)
+        chal = np.zeros((max(self.num_vars - 1, 1), L), dtype=np.uint32)
+        cs = ctypes.c_uint64()
+        _lib.check(self.lib.sc_proof_result(self.handle, _lib.as_u32p(polys), _lib.as_u32p(chal),
+                                            ctypes.byref(cs)))
+        return polys, chal[: self.num_vars - 1], int(cs.value)
+
+    def tables(self) -> np.ndarray:
+        out = np.empty((self.comp.num_tables, 1 << self.num_vars, self.field.limbs), dtype=np.uint32)
+        _lib.check(self.lib.sc_proof_tables(self.handle, _lib.as_u32p(out)))
+        return out
+
+    def info(self) -> dict:
+        d, pr, nb = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_uint64()
+        _lib.check(self.lib.sc_proof_info(self.handle, ctypes.byref(d), ctypes.byref(pr), ctypes.byref(nb)))
+        return {"degree": d.value, "pass_rounds": pr.value, "device_bytes": nb.value}
+
+    def set_profiling(self, enable: bool = True) -> None:
+        _lib.check(self.lib.sc_proof_set_profiling(self.handle, 1 if enable else 0))
+
+    def kernel_times(self) -> tuple[list[float], float]:
+        """(pass kernel ms per fused round, tail kernel ms) of the last profiled run."""
+        rounds = self.info()["pass_rounds"]
+        buf = (ctypes.c_float * max(rounds, 1))()
+        tail = ctypes.c_float()
+        _lib.check(self.lib.sc_proof_kernel_times(self.handle, buf, rounds, ctypes.byref(tail)))
+        return [float(buf[j]) for j in range(rounds)], float(tail.value)
+
+    def canonical_result(self) -> tuple[list[list[int]], list[int], int]:
+        """Round polynomials and challenges decoded to canonical ints, plus the checksum."""
+        polys, chal, cs = self.result()
+        f = self.field
+        g = [[f.montgomery_decode(v) for v in f.limbs_to_ints(polys[j])] for j in range(self.num_vars)]
+        r = [f.montgomery_decode(v) for v in f.limbs_to_ints(chal)] if self.num_vars > 1 else []
+        return g, r, cs
