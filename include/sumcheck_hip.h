@@ -110,6 +110,11 @@ int sc_proof_tables(sc_proof* proof, uint32_t* out);
  * bytes held. Any pointer may be NULL. */
 int sc_proof_info(sc_proof* proof, int32_t* degree, int32_t* pass_rounds, uint64_t* device_bytes);
 
+/* Compositions with compiled kernels (a*b, a*b*c, a*b*c + d*e) run them by default; enable = 0
+ * forces the interpreted kernels (same results). Returns the compiled composition id that
+ * matches (0: none), or a negative error code. */
+int sc_proof_use_compiled(sc_proof* proof, int32_t enable);
+
 /* Per-kernel timing: when enabled, eager runs record a hipEvent pair around every pass
  * kernel and around the tail kernel, on the stream they run on. sc_proof_kernel_times then
  * returns, for the last run, pass_ms[j] for rounds j < pass_rounds and *tail_ms. */
@@ -134,6 +139,12 @@ int sc_field_op(sc_context* ctx, int32_t bit_width, int32_t op, uint64_t n, cons
  * r_mont [L] out (field_sumcheck.cu:857-907 computed on the GPU). */
 int sc_transcript_challenge(sc_context* ctx, int32_t bit_width, int32_t round, const uint32_t* g_mont,
                             int32_t npts, uint32_t* r_mont);
+
+/* Measured sustained lane rates (operations per second) of the VALU operations the field
+ * arithmetic is built from: rates[0] v_mad_u64_u32 (32x32+64), rates[1] v_mul_hi_u32,
+ * rates[2] 64-bit add with carry (v_add_co + v_addc), rates[3] v_fma_f64. The INT32
+ * roofline of the wide fields is priced against rates[0]. n <= 4. */
+int sc_calibrate(sc_context* ctx, double* rates, int32_t n);
 
 #ifdef __cplusplus
 }

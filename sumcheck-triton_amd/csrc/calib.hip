@@ -1,0 +1,117 @@
+// Throughput calibration kernels: the sustained lane rate of the VALU operations the field
+// arithmetic is built from, measured on the device that runs the prover. The INT32 roofline
+// of the wide (128/256-bit) fields is priced against the measured v_mad_u64_u32 rate.
+//
+// Each thread runs CHAINS independent dependency chains for `iters` iterations, so the
+// measured rate is issue-bound rather than latency-bound. Results are stored so no chain is
+// dead code.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "calib.hpp"
+
+namespace sc {
+
+constexpr int CHAINS = 8;
+
+__global__ void __launch_bounds__(256) calib_mad_u64(uint32_t b, int iters, uint64_t* out) {
+  uint64_t x[CHAINS];
+#pragma unroll
+  for (int k = 0; k < CHAINS; ++k) x[k] = threadIdx.x + blockIdx.x * 7919u + k;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < CHAINS; ++k) x[k] = (uint64_t)(uint32_t)x[k] * b + x[k];
+  }
+  uint64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < CHAINS; ++k) s ^= x[k];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(256) calib_mul_hi_u32(uint32_t b, int iters, uint32_t* out) {
+  uint32_t x[CHAINS];
+#pragma unroll
+  for (int k = 0; k < CHAINS; ++k) x[k] = threadIdx.x * 31u + k + 1u;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < CHAINS; ++k) x[k] = __umulhi(x[k], b) + x[k];
+  }
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < CHAINS; ++k) s ^= x[k];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(256) calib_add_co(uint32_t b, int iters, uint32_t* out) {
+  uint64_t x[CHAINS];
+#pragma unroll
+  for (int k = 0; k < CHAINS; ++k) x[k] = threadIdx.x + k;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < CHAINS; ++k) x[k] = x[k] + x[(k + 3) % CHAINS] + b;  // 2x (add_co + addc)
+  }
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < CHAINS; ++k) s ^= (uint32_t)x[k] ^ (uint32_t)(x[k] >> 32);
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(256) calib_fma_f64(double b, int iters, double* out) {
+  double x[CHAINS];
+#pragma unroll
+  for (int k = 0; k < CHAINS; ++k) x[k] = (double)(threadIdx.x + k);
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < CHAINS; ++k) x[k] = __fma_rn(x[k], b, 0.5);
+  }
+  double s = 0;
+#pragma unroll
+  for (int k = 0; k < CHAINS; ++k) s += x[k];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+hipError_t calibrate(hipStream_t s, double* rates, int n) {
+  // grid: 8 blocks per CU of a 256-CU device, 256 threads each
+  const int blocks = 2048, threads = 256, iters = 4096;
+  void* buf = nullptr;
+  hipError_t e = hipMalloc(&buf, (size_t)blocks * threads * 8);
+  if (e != hipSuccess) return e;
+  hipEvent_t a = nullptr, b = nullptr;
+  e = hipEventCreate(&a);
+  if (e == hipSuccess) e = hipEventCreate(&b);
+  for (int k = 0; k < n && k < CALIB_COUNT && e == hipSuccess; ++k) {
+    for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {  // rep 0 warms up
+      e = hipEventRecord(a, s);
+      if (e != hipSuccess) break;
+      switch (k) {
+        case CALIB_MAD_U64:
+          hipLaunchKernelGGL(calib_mad_u64, dim3(blocks), dim3(threads), 0, s, 0x9E3779B9u, iters, (uint64_t*)buf);
+          break;
+        case CALIB_MUL_HI_U32:
+          hipLaunchKernelGGL(calib_mul_hi_u32, dim3(blocks), dim3(threads), 0, s, 0x9E3779B9u, iters, (uint32_t*)buf);
+          break;
+        case CALIB_ADD_CO:
+          hipLaunchKernelGGL(calib_add_co, dim3(blocks), dim3(threads), 0, s, 0x9E3779B9u, iters, (uint32_t*)buf);
+          break;
+        default:
+          hipLaunchKernelGGL(calib_fma_f64, dim3(blocks), dim3(threads), 0, s, 1.0000001, iters, (double*)buf);
+          break;
+      }
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipEventRecord(b, s);
+      if (e == hipSuccess) e = hipEventSynchronize(b);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+      // the 64-bit add kernel does two adds per chain step
+      const double per_step = k == CALIB_ADD_CO ? 2.0 : 1.0;
+      if (e == hipSuccess) rates[k] = per_step * blocks * threads * iters * CHAINS / (ms * 1e-3);
+    }
+  }
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  (void)hipFree(buf);
+  return e;
+}
+
+}  // namespace sc

@@ -142,7 +142,7 @@ SC_HD Elem<F::L> sub(const Elem<F::L>& a, const Elem<F::L>& b) {
 
 // CIOS Montgomery product a*b*R^-1 mod p; a, b < p (a < R suffices when b < p).
 template <class F>
-SC_HD Elem<F::L> mont_mul(const Elem<F::L>& a, const Elem<F::L>& b) {
+SC_HD Elem<F::L> mont_mul_cios(const Elem<F::L>& a, const Elem<F::L>& b) {
   constexpr int L = F::L;
   uint32_t t[L + 2];
 #pragma unroll
@@ -182,6 +182,163 @@ SC_HD Elem<F::L> mont_mul(const Elem<F::L>& a, const Elem<F::L>& b) {
 #pragma unroll
   for (int i = 0; i < L; ++i) r.v[i] = use_q ? q.v[i] : r.v[i];
   return r;
+}
+
+// acc (64-bit) + top (32-bit) += a * b. On the device this is v_mad_u64_u32 with its carry
+// out, then v_addc_co_u32 into top: two instructions per multiply-accumulate, with the
+// accumulator pair used in place as the 64-bit addend (a C++ formulation makes the compiler
+// re-pair zero-extended halves with extra moves). Arithmetic only, no memory operands.
+SC_HD void mac(uint64_t& acc, uint32_t& top, uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint64_t carry;
+  asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\t"
+      "v_addc_co_u32_e64 %2, %1, 0, %2, %1"
+      : "+v"(acc), "=&s"(carry), "+v"(top)
+      : "v"(a), "v"(b));
+#else
+  const uint64_t s = (uint64_t)a * b + acc;
+  top += s < acc ? 1u : 0u;
+  acc = s;
+#endif
+}
+
+// Montgomery product by finely integrated product scanning (Koc, Acar, Kaliski, "Analyzing
+// and comparing Montgomery multiplication algorithms", 1996): column k of a*b and of m*p is
+// summed into a 96-bit accumulator, m_k chosen so the column's low word vanishes. 2L^2 + L
+// multiply-accumulates; the result is < 2p and is brought into [0, p) by one subtraction.
+template <class F>
+SC_HD Elem<F::L> mont_mul_fips(const Elem<F::L>& a, const Elem<F::L>& b) {
+  constexpr int L = F::L;
+  uint32_t m[L];
+  uint64_t acc = 0;
+  uint32_t top = 0;
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+#pragma unroll
+    for (int j = 0; j < i; ++j) {
+      mac(acc, top, a.v[j], b.v[i - j]);
+      mac(acc, top, m[j], F::p(i - j));
+    }
+    mac(acc, top, a.v[i], b.v[0]);
+    m[i] = (uint32_t)acc * F::NPRIME;
+    mac(acc, top, m[i], F::p(0));
+    acc = (acc >> 32) | ((uint64_t)top << 32);
+    top = 0;
+  }
+  Elem<L> r;
+#pragma unroll
+  for (int i = L; i < 2 * L; ++i) {
+#pragma unroll
+    for (int j = i - L + 1; j < L; ++j) {
+      mac(acc, top, a.v[j], b.v[i - j]);
+      mac(acc, top, m[j], F::p(i - j));
+    }
+    r.v[i - L] = (uint32_t)acc;
+    acc = (acc >> 32) | ((uint64_t)top << 32);
+    top = 0;
+  }
+  Elem<L> q;
+  const uint32_t borrow = sub_p<F>(q, r);
+  const bool use_q = (acc != 0u) | (borrow ^ 1u);
+#pragma unroll
+  for (int i = 0; i < L; ++i) r.v[i] = use_q ? q.v[i] : r.v[i];
+  return r;
+}
+
+// Goldilocks (p = 2^64 - 2^32 + 1, R = 2^64). 2^96 = -1 and 2^192 = 1 mod p, so
+// R^-1 = 2^128 = -2^32 and, for x = a*b = x3:x2:x1:x0 (32-bit words),
+//   x * R^-1 = (x3 - x1 - x0) * 2^32 + x1 + x2   (mod p).
+// The 64-bit sums below fold each wrap-around back in with 2^64 = 2^32 - 1 (mod p).
+// Requires a, b < p (any b < 2^64 with a < p keeps x < 2^128).
+SC_HD Elem<2> goldilocks_mont_mul(const Elem<2>& a, const Elem<2>& b) {
+  const uint64_t p00 = (uint64_t)a.v[0] * b.v[0];
+  const uint64_t t0 = (uint64_t)a.v[0] * b.v[1] + (p00 >> 32);
+  const uint64_t t1 = (uint64_t)a.v[1] * b.v[0] + (uint32_t)t0;
+  const uint64_t hi = (uint64_t)a.v[1] * b.v[1] + ((t0 >> 32) + (t1 >> 32));
+  const uint64_t x0 = (uint32_t)p00, x1 = (uint32_t)t1;
+  // s = x3 * 2^32 + x2 + x1 (hi is x3 * 2^32 + x2)
+  const uint64_t add = hi + x1;
+  const uint64_t s = add + ((add < hi) ? 0xFFFFFFFFull : 0ull);
+  // s -= x0 * 2^32, s -= x1 * 2^32; a borrow of 2^64 is worth -(2^32 - 1)
+  uint64_t d = s - (x0 << 32);
+  d -= (d > s) ? 0xFFFFFFFFull : 0ull;
+  const uint64_t e = d - (x1 << 32);
+  const uint64_t f = e - ((e > d) ? 0xFFFFFFFFull : 0ull);
+  const uint64_t r = f >= 0xFFFFFFFF00000001ull ? f - 0xFFFFFFFF00000001ull : f;
+  Elem<2> out;
+  out.v[0] = (uint32_t)r;
+  out.v[1] = (uint32_t)(r >> 32);
+  return out;
+}
+
+// Mersenne p = 2^127 - 1, R = 2^128: 2^127 = 1, so R = 2 and R^-1 = 2^126 (mod p), and
+// x * R^-1 is x mod p rotated right by one bit within 127 bits. x mod p folds the product
+// at bit 127 twice (x < 2^254).
+SC_HD Elem<4> mersenne127_mont_mul(const Elem<4>& a, const Elem<4>& b) {
+  uint32_t t[8];
+  uint64_t acc = 0;
+  uint32_t top = 0;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+#pragma unroll
+    for (int i = (k < 4 ? 0 : k - 3); i <= (k < 4 ? k : 3); ++i) mac(acc, top, a.v[i], b.v[k - i]);
+    t[k] = (uint32_t)acc;
+    acc = (acc >> 32) | ((uint64_t)top << 32);
+    top = 0;
+  }
+  t[7] = (uint32_t)acc;
+  // lo = x mod 2^127, hi = x >> 127 (both < 2^127)
+  uint32_t hi[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) hi[i] = (t[i + 3] >> 31) | (t[i + 4] << 1);
+  uint32_t lo[4] = {t[0], t[1], t[2], t[3] & 0x7FFFFFFFu};
+  // s = lo + hi < 2^128; fold bit 127 once more (s < 2^127 + 1 afterwards: s <= p)
+  uint32_t sum[4];
+  uint64_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    c += (uint64_t)lo[i] + hi[i];
+    sum[i] = (uint32_t)c;
+    c >>= 32;
+  }
+  const uint32_t bit127 = sum[3] >> 31;
+  sum[3] &= 0x7FFFFFFFu;
+  c = bit127;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    c += sum[i];
+    sum[i] = (uint32_t)c;
+    c >>= 32;
+  }
+  // s == p (all 127 bits set) is 0 mod p
+  const bool is_p = (sum[0] & sum[1] & sum[2]) == 0xFFFFFFFFu && sum[3] == 0x7FFFFFFFu;
+  // rotate right by one within 127 bits: (s >> 1) | ((s & 1) << 126)
+  Elem<4> out;
+  out.v[0] = (sum[0] >> 1) | (sum[1] << 31);
+  out.v[1] = (sum[1] >> 1) | (sum[2] << 31);
+  out.v[2] = (sum[2] >> 1) | (sum[3] << 31);
+  out.v[3] = (sum[3] >> 1) | ((sum[0] & 1u) << 30);
+  if (is_p) {
+    out.v[0] = out.v[1] = out.v[2] = out.v[3] = 0u;
+  }
+  return out;
+}
+
+// Montgomery product a*b*R^-1 mod p in [0, p) for a, b in [0, p). Field-specific
+// reductions where the modulus has structure; the result is the same unique value.
+template <class F>
+SC_HD Elem<F::L> mont_mul(const Elem<F::L>& a, const Elem<F::L>& b) {
+  if constexpr (F::WIDTH == 64) {
+    return goldilocks_mont_mul(a, b);
+  } else if constexpr (F::WIDTH == 128) {
+    return mersenne127_mont_mul(a, b);
+  } else {
+#if defined(SC_MONT_CIOS)
+    return mont_mul_cios<F>(a, b);
+#else
+    return mont_mul_fips<F>(a, b);
+#endif
+  }
 }
 
 template <class F>

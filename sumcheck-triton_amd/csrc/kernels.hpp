@@ -112,14 +112,60 @@ __device__ __forceinline__ void compose_add(Elem<F::L>& acc, const Elem<F::L> (&
   }
 }
 
+// Compiled composition STATIC_COMPS[SID]: every loop has constant bounds and indices, so it
+// unrolls into straight-line field operations on fixed registers.
+template <class F, int SID, int TMAX>
+__device__ __forceinline__ void compose_static(Elem<F::L>& acc, const Elem<F::L> (&x)[TMAX],
+                                               const CompDesc* __restrict__ cd) {
+  constexpr StaticComp C = STATIC_COMPS[SID];
+  static_assert(C.num_tables <= TMAX, "table slots");
+#pragma unroll
+  for (int k = 0; k < C.num_terms; ++k) {
+    Elem<F::L> prod = x[C.vars[k][0]];
+#pragma unroll
+    for (int m = 1; m < C.lens[k]; ++m) prod = mont_mul<F>(prod, x[C.vars[k][m]]);
+    if (C.kinds[k] == COEFF_ONE) {
+      acc = add<F>(acc, prod);
+    } else if (C.kinds[k] == COEFF_MINUS_ONE) {
+      acc = sub<F>(acc, prod);
+    } else {
+      acc = add<F>(acc, mont_mul<F>(prod, load_elem<F::L>(cd->coeff[k])));
+    }
+  }
+}
+
+template <class F, int SID, int TMAX>
+__device__ __forceinline__ void compose(Elem<F::L>& acc, const Elem<F::L> (&x)[TMAX],
+                                        const CompDesc* __restrict__ cd, int K) {
+  if constexpr (SID == 0) {
+    compose_add<F, TMAX>(acc, x, cd, K);
+  } else {
+    compose_static<F, SID, TMAX>(acc, x, cd);
+  }
+}
+
+// Shape of the composition a kernel runs: compile-time for compiled ones.
+template <int SID>
+struct Shape {
+  __device__ static int tables(const CompDesc* cd) {
+    if constexpr (SID == 0) return cd->num_tables; else return STATIC_COMPS[SID].num_tables;
+  }
+  __device__ static int terms(const CompDesc* cd) {
+    if constexpr (SID == 0) return cd->num_terms; else return STATIC_COMPS[SID].num_terms;
+  }
+  __device__ static int npts(const CompDesc* cd) {
+    if constexpr (SID == 0) return cd->npts; else return static_degree(SID) + 1;
+  }
+};
+
 // cur holds f[2i] and oth holds f[2i+1] of every table on entry. Accumulates
 // C(f[2i] + t (f[2i+1] - f[2i])) into acc[t] for t < npts, with t >= 2 reached by repeated
 // addition of the difference (reference.py:89-99).
-template <class F, int TMAX, int PMAX>
+template <class F, int TMAX, int PMAX, int SID>
 __device__ __forceinline__ void eval_points(Elem<F::L> (&cur)[TMAX], Elem<F::L> (&oth)[TMAX],
                                             Elem<F::L> (&acc)[PMAX], const CompDesc* __restrict__ cd,
                                             int T, int K, int npts) {
-  compose_add<F, TMAX>(acc[0], cur, cd, K);
+  compose<F, SID, TMAX>(acc[0], cur, cd, K);
 #pragma unroll
   for (int v = 0; v < TMAX; ++v) {
     if (v < T) {
@@ -128,7 +174,7 @@ __device__ __forceinline__ void eval_points(Elem<F::L> (&cur)[TMAX], Elem<F::L> 
       cur[v] = o;
     }
   }
-  compose_add<F, TMAX>(acc[1], cur, cd, K);
+  compose<F, SID, TMAX>(acc[1], cur, cd, K);
 #pragma unroll
   for (int t = 2; t < PMAX; ++t) {
     if (t < npts) {
@@ -136,7 +182,7 @@ __device__ __forceinline__ void eval_points(Elem<F::L> (&cur)[TMAX], Elem<F::L> 
       for (int v = 0; v < TMAX; ++v) {
         if (v < T) cur[v] = add<F>(cur[v], oth[v]);
       }
-      compose_add<F, TMAX>(acc[t], cur, cd, K);
+      compose<F, SID, TMAX>(acc[t], cur, cd, K);
     }
   }
 }
@@ -183,9 +229,9 @@ __device__ __forceinline__ void load_pair(Elem<F::L> (&cur)[TMAX], Elem<F::L> (&
 // Reductions (exact modular sums, so any order gives the same bits)
 // ---------------------------------------------------------------------------------------------
 
-// Sums acc over the block into out[t*L + limb] (LDS) for t < npts. red needs
-// (BLOCK/64)*PMAX*L words. Ends with a barrier.
-template <class F, int PMAX>
+// Sums acc over the NT-thread block into out[t*L + limb] (LDS) for t < npts. red needs
+// (NT/64)*PMAX*L words. Ends with a barrier.
+template <class F, int PMAX, int NT>
 __device__ __forceinline__ void block_reduce(Elem<F::L> (&acc)[PMAX], int npts, uint32_t* red,
                                              uint32_t* out) {
   constexpr int L = F::L;
@@ -212,7 +258,7 @@ __device__ __forceinline__ void block_reduce(Elem<F::L> (&acc)[PMAX], int npts, 
     const int t = threadIdx.x;
     Elem<L> s = load_elem<L>(red + t * L);
 #pragma unroll
-    for (int w = 1; w < BLOCK / 64; ++w) s = add<F>(s, load_elem<L>(red + (w * PMAX + t) * L));
+    for (int w = 1; w < NT / 64; ++w) s = add<F>(s, load_elem<L>(red + (w * PMAX + t) * L));
 #pragma unroll
     for (int k = 0; k < L; ++k) out[t * L + k] = s.v[k];
   }
@@ -242,8 +288,8 @@ __device__ __forceinline__ void sha3_uniform(const uint32_t* msg, int nblocks, u
 
 // Publishes g (LDS, npts*L words) as round `round`'s polynomial and, unless it is the last
 // round, derives r_round into r_out (LDS, L words) and st->r[round]. All threads call it.
-template <class F>
-__device__ void finish_round(const uint32_t* g, int npts, int round, int num_rounds, int mode,
+template <class F, int NT>
+__device__ __forceinline__ void finish_round(const uint32_t* g, int npts, int round, int num_rounds, int mode,
                              const FieldConsts* __restrict__ fc, ProofState* st,
                              const uint32_t* __restrict__ explicit_r, uint32_t* msg, uint32_t* r_out) {
   constexpr int L = F::L;
@@ -255,7 +301,7 @@ __device__ void finish_round(const uint32_t* g, int npts, int round, int num_rou
     // Montgomery-encoded (field_sumcheck.cu:857-907): sum_i digest_i * 2^(32i) * R mod p.
     const int nvals = npts * L;
     const int nb = transcript_blocks(nvals);
-    for (int k = tid; k < 34 * nb; k += BLOCK) msg[k] = transcript_word(k, (uint32_t)round, g, nvals, nb);
+    for (int k = tid; k < 34 * nb; k += NT) msg[k] = transcript_word(k, (uint32_t)round, g, nvals, nb);
     __syncthreads();
     if (tid < 64) {
       uint32_t digest[8];
@@ -290,15 +336,15 @@ __device__ void finish_round(const uint32_t* g, int npts, int round, int num_rou
 // Kernels
 // ---------------------------------------------------------------------------------------------
 
-template <class F, int TMAX, int PMAX>
+template <class F, int TMAX, int PMAX, int SID>
 __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
   constexpr int L = F::L;
   __shared__ uint32_t s_red[(BLOCK / 64) * PMAX * L];
   __shared__ uint32_t s_g[PMAX * L];
   const CompDesc* __restrict__ cd = a.comp;
-  const int T = cd->num_tables;
-  const int K = cd->num_terms;
-  const int npts = cd->npts;
+  const int T = Shape<SID>::tables(cd);
+  const int K = Shape<SID>::terms(cd);
+  const int npts = Shape<SID>::npts(cd);
   const bool fold = a.fold != 0;
 
   Elem<L> acc[PMAX];
@@ -311,9 +357,9 @@ __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
   for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.pairs; i += stride) {
     Elem<L> cur[TMAX], oth[TMAX];
     load_pair<F, TMAX>(cur, oth, a.src, a.dst, a.src_n, i, T, fold, r);
-    eval_points<F, TMAX, PMAX>(cur, oth, acc, cd, T, K, npts);
+    eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts);
   }
-  block_reduce<F, PMAX>(acc, npts, s_red, s_g);
+  block_reduce<F, PMAX, BLOCK>(acc, npts, s_red, s_g);
   if (threadIdx.x < npts * L) {
     a.partials[(uint64_t)blockIdx.x * (MAX_POINTS * L) + threadIdx.x] = s_g[threadIdx.x];
   }
@@ -338,21 +384,23 @@ __global__ void __launch_bounds__(BLOCK) finalize_kernel(FinalizeArgs a) {
       if (t < npts) acc[t] = add<F>(acc[t], load_elem<L>(a.partials + ((uint64_t)b * MAX_POINTS + t) * L));
     }
   }
-  block_reduce<F, PMAX>(acc, npts, s_red, s_g);
-  finish_round<F>(s_g, npts, a.round, a.num_rounds, a.challenge_mode, a.fc, a.st, a.explicit_r, s_msg, s_r);
+  block_reduce<F, PMAX, BLOCK>(acc, npts, s_red, s_g);
+  finish_round<F, BLOCK>(s_g, npts, a.round, a.num_rounds, a.challenge_mode, a.fc, a.st, a.explicit_r, s_msg, s_r);
 }
 
-template <class F, int TMAX, int PMAX>
-__global__ void __launch_bounds__(BLOCK) tail_kernel(TailArgs a) {
+// One workgroup of TB threads (more than a pass block: the tail's rounds have no other
+// workgroups to overlap with, so it uses a whole CU).
+template <class F, int TMAX, int PMAX, int SID, int TB>
+__global__ void __launch_bounds__(TB) tail_kernel(TailArgs a) {
   constexpr int L = F::L;
-  __shared__ uint32_t s_red[(BLOCK / 64) * PMAX * L];
+  __shared__ uint32_t s_red[(TB / 64) * PMAX * L];
   __shared__ uint32_t s_g[PMAX * L];
   __shared__ uint32_t s_msg[3 * 34];
   __shared__ uint32_t s_r[L];
   const CompDesc* __restrict__ cd = a.comp;
-  const int T = cd->num_tables;
-  const int K = cd->num_terms;
-  const int npts = cd->npts;
+  const int T = Shape<SID>::tables(cd);
+  const int K = Shape<SID>::terms(cd);
+  const int npts = Shape<SID>::npts(cd);
 
   if (a.first_round > 0 && threadIdx.x < L) s_r[threadIdx.x] = a.st->r[a.first_round - 1][threadIdx.x];
   __syncthreads();
@@ -366,13 +414,13 @@ __global__ void __launch_bounds__(BLOCK) tail_kernel(TailArgs a) {
     Elem<L> acc[PMAX];
 #pragma unroll
     for (int t = 0; t < PMAX; ++t) acc[t] = zero<F>();
-    for (uint64_t i = threadIdx.x; i < pairs; i += BLOCK) {
+    for (uint64_t i = threadIdx.x; i < pairs; i += TB) {
       Elem<L> cur[TMAX], oth[TMAX];
       load_pair<F, TMAX>(cur, oth, src, dst, src_n, i, T, fold, r);
-      eval_points<F, TMAX, PMAX>(cur, oth, acc, cd, T, K, npts);
+      eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts);
     }
-    block_reduce<F, PMAX>(acc, npts, s_red, s_g);
-    finish_round<F>(s_g, npts, j, a.num_rounds, a.challenge_mode, a.fc, a.st, a.explicit_r, s_msg, s_r);
+    block_reduce<F, PMAX, TB>(acc, npts, s_red, s_g);
+    finish_round<F, TB>(s_g, npts, j, a.num_rounds, a.challenge_mode, a.fc, a.st, a.explicit_r, s_msg, s_r);
     __syncthreads();
   }
 }
@@ -441,7 +489,7 @@ __global__ void __launch_bounds__(BLOCK) transcript_kernel(const uint32_t* g, in
   __shared__ uint32_t s_r[L];
   if (threadIdx.x < npts * L) s_g[threadIdx.x] = g[threadIdx.x];
   __syncthreads();
-  finish_round<F>(s_g, npts, round, round + 2, CHALLENGE_SHA3, fc, st, nullptr, s_msg, s_r);
+  finish_round<F, BLOCK>(s_g, npts, round, round + 2, CHALLENGE_SHA3, fc, st, nullptr, s_msg, s_r);
 }
 
 }  // namespace sc

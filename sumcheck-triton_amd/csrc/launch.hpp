@@ -11,9 +11,10 @@ namespace sc {
 struct Launchers {
   int limbs;
   hipError_t (*fill)(hipStream_t, const FillArgs&);
-  hipError_t (*pass)(hipStream_t, int tmax, int pmax, const PassArgs&, unsigned grid);
+  // sid: compiled composition id (STATIC_COMPS), 0 = interpreted with capacities tmax/pmax
+  hipError_t (*pass)(hipStream_t, int sid, int tmax, int pmax, const PassArgs&, unsigned grid);
   hipError_t (*finalize)(hipStream_t, int pmax, const FinalizeArgs&);
-  hipError_t (*tail)(hipStream_t, int tmax, int pmax, const TailArgs&);
+  hipError_t (*tail)(hipStream_t, int sid, int tmax, int pmax, const TailArgs&);
   hipError_t (*field_op)(hipStream_t, int op, uint64_t n, const uint32_t* a, const uint32_t* b,
                          uint32_t* out, const FieldConsts* fc);
   hipError_t (*transcript)(hipStream_t, const uint32_t* g, int npts, int round,

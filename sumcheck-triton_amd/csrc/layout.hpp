@@ -22,6 +22,31 @@ constexpr int BLOCK = 256;
 constexpr int MAX_PASS_BLOCKS = 4096;
 
 enum CoeffKind : int32_t { COEFF_ONE = 0, COEFF_MINUS_ONE = 1, COEFF_GENERAL = 2 };
+
+// Compositions with compiled kernels: the round kernels unroll these at compile time
+// (static table slots, no term interpretation). Any other composition runs through the
+// interpreted kernels. The host selects a compiled one when the composition's tables,
+// terms (in order), factors and coefficient kinds match exactly.
+struct StaticComp {
+  int num_tables;
+  int num_terms;
+  int lens[4];
+  int vars[4][8];
+  int kinds[4];
+};
+constexpr int NUM_STATIC_COMPS = 4;  // id 0 = interpreted
+constexpr StaticComp STATIC_COMPS[NUM_STATIC_COMPS] = {
+    {0, 0, {0}, {{0}}, {0}},
+    {2, 1, {2}, {{0, 1}}, {COEFF_ONE}},                              // a*b (zk_spartan_2 shape)
+    {3, 1, {3}, {{0, 1, 2}}, {COEFF_ONE}},                           // a*b*c (poly_abc)
+    {5, 2, {3, 2}, {{0, 1, 2}, {3, 4}}, {COEFF_ONE, COEFF_ONE}},     // a*b*c + d*e (poly_abc_plus_de)
+};
+constexpr int static_degree(int id) {
+  int d = 0;
+  for (int k = 0; k < STATIC_COMPS[id].num_terms; ++k)
+    if (STATIC_COMPS[id].lens[k] > d) d = STATIC_COMPS[id].lens[k];
+  return d;
+}
 enum ChallengeMode : int32_t { CHALLENGE_FIXED = 0, CHALLENGE_SHA3 = 1, CHALLENGE_EXPLICIT = 2 };
 enum FillKind : int32_t { FILL_AFFINE = 0, FILL_UNIFORM = 1 };
 

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/sumcheck_hip.h"
+#include "calib.hpp"
 #include "field.hpp"
 #include "launch.hpp"
 #include "layout.hpp"
@@ -162,6 +163,27 @@ void consts_any(int fidx, FieldConsts& fc) {
   }
 }
 
+// Compiled composition matching d exactly (same tables, terms in order, factor counts and
+// coefficient kinds), or 0.
+int match_static(const CompDesc& d) {
+  for (int id = 1; id < NUM_STATIC_COMPS; ++id) {
+    const StaticComp& c = STATIC_COMPS[id];
+    if (c.num_tables != d.num_tables || c.num_terms != d.num_terms) continue;
+    bool ok = true;
+    for (int k = 0; k < c.num_terms && ok; ++k) {
+      uint32_t exps[MAX_TABLES / 8] = {0};
+      for (int m = 0; m < c.lens[k]; ++m) {
+        const int v = c.vars[k][m];
+        exps[v >> 3] += 1u << ((v & 7) * 4);
+      }
+      for (int w = 0; w < MAX_TABLES / 8; ++w) ok = ok && exps[w] == d.exps[k][w];
+      ok = ok && c.kinds[k] == d.kind[k];
+    }
+    if (ok) return id;
+  }
+  return 0;
+}
+
 int pick_tmax(int T) {
   for (int t : TMAX_CHOICES)
     if (T <= t) return t;
@@ -180,6 +202,8 @@ struct sc_context {
 struct sc_proof {
   sc_context* ctx = nullptr;
   int fidx = 0, L = 0, T = 0, npts = 0, num_vars = 0, tmax = 0, pmax = 0;
+  int sid = 0;             // compiled composition in use (0 = interpreted)
+  int matched_sid = 0;     // compiled composition that matches, if any
   const Launchers* ln = nullptr;
   uint64_t n0 = 0;
   uint32_t* d_tables = nullptr;
@@ -248,7 +272,7 @@ int enqueue_rounds(sc_proof* p, int mode) {
     uint64_t g = (a.pairs + BLOCK - 1) / BLOCK;
     if (g > (uint64_t)p->max_pass_blocks) g = (uint64_t)p->max_pass_blocks;
     if (p->profiling) SC_HIP(hipEventRecord(p->prof_ev[2 * j], s));
-    SC_HIP(p->ln->pass(s, p->tmax, p->pmax, a, (unsigned)g));
+    SC_HIP(p->ln->pass(s, p->sid, p->tmax, p->pmax, a, (unsigned)g));
     if (p->profiling) SC_HIP(hipEventRecord(p->prof_ev[2 * j + 1], s));
     FinalizeArgs f;
     f.partials = p->d_partials;
@@ -277,7 +301,7 @@ int enqueue_rounds(sc_proof* p, int mode) {
     t.st = p->d_state;
     t.explicit_r = p->d_explicit;
     if (p->profiling) SC_HIP(hipEventRecord(p->prof_ev[2 * MAX_ROUNDS], s));
-    SC_HIP(p->ln->tail(s, p->tmax, p->pmax, t));
+    SC_HIP(p->ln->tail(s, p->sid, p->tmax, p->pmax, t));
     if (p->profiling) SC_HIP(hipEventRecord(p->prof_ev[2 * MAX_ROUNDS + 1], s));
   }
   return SC_OK;
@@ -411,6 +435,8 @@ int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* co
   p->pmax = p->npts <= 4 ? 4 : 8;
   p->n0 = (uint64_t)1 << num_vars;
   p->tail_pairs = default_tail_pairs(p->L);
+  p->matched_sid = match_static(desc);
+  p->sid = p->matched_sid;
 
   const uint64_t words = (uint64_t)p->T * p->n0 * p->L;
   auto cleanup = [&](int code) {
@@ -597,6 +623,20 @@ int sc_proof_info(sc_proof* p, int32_t* degree, int32_t* pass_rounds, uint64_t* 
   return SC_OK;
 }
 
+int sc_proof_use_compiled(sc_proof* p, int32_t enable) {
+  if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
+  const int sid = enable ? p->matched_sid : 0;
+  if (sid != p->sid && p->graph_exec) {
+    (void)hipGraphExecDestroy(p->graph_exec);
+    (void)hipGraphDestroy(p->graph);
+    p->graph_exec = nullptr;
+    p->graph = nullptr;
+    p->graph_mode = -1;
+  }
+  p->sid = sid;
+  return p->matched_sid;
+}
+
 int sc_proof_set_profiling(sc_proof* p, int32_t enable) {
   if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
   SC_HIP(hipSetDevice(p->ctx->device));
@@ -693,6 +733,15 @@ int sc_transcript_challenge(sc_context* ctx, int32_t bit_width, int32_t round, c
   (void)hipFree(d_g);
   if (d_st) (void)hipFree(d_st);
   if (e != hipSuccess) return fail(SC_ERR_HIP, std::string("sc_transcript_challenge: ") + hipGetErrorString(e));
+  return SC_OK;
+}
+
+int sc_calibrate(sc_context* ctx, double* rates, int32_t n) {
+  if (ctx == nullptr || rates == nullptr) return fail(SC_ERR_ARG, "NULL argument");
+  if (n < 0 || n > CALIB_COUNT) return fail(SC_ERR_ARG, "n must be in [0, 4]");
+  SC_HIP(hipSetDevice(ctx->device));
+  for (int k = 0; k < n; ++k) rates[k] = 0.0;
+  SC_HIP(calibrate(ctx->stream, rates, n));
   return SC_OK;
 }
 

@@ -75,6 +75,7 @@ SIGNATURES = [
     ("sc_proof_result", ctypes.c_int, [ctypes.c_void_p, u32p, u32p, u64p]),
     ("sc_proof_tables", ctypes.c_int, [ctypes.c_void_p, u32p]),
     ("sc_proof_info", ctypes.c_int, [ctypes.c_void_p, i32p, i32p, u64p]),
+    ("sc_proof_use_compiled", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("sc_proof_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("sc_proof_kernel_times", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
@@ -84,6 +85,7 @@ SIGNATURES = [
       ctypes.POINTER(ScInput), ctypes.c_int32, u32p, u32p, u32p, u64p]),
     ("sc_field_op", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, u32p, u32p, u32p]),
+    ("sc_calibrate", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int32]),
     ("sc_transcript_challenge", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, u32p, ctypes.c_int32, u32p]),
 ]

@@ -126,6 +126,12 @@ class Context:
                                         _lib.as_u32p(a), _lib.as_u32p(bb), _lib.as_u32p(out)))
         return out
 
+    def calibrate(self) -> dict[str, float]:
+        """Measured lane rates (ops/s): mad_u64_u32, mul_hi_u32, add64 (add_co+addc), fma_f64."""
+        rates = (ctypes.c_double * 4)()
+        _lib.check(self.lib.sc_calibrate(self.handle, rates, 4))
+        return {"mad_u64_u32": rates[0], "mul_hi_u32": rates[1], "add64": rates[2], "fma_f64": rates[3]}
+
     def transcript_challenge(self, bit_width: int, round_idx: int, g_mont: np.ndarray) -> np.ndarray:
         """Device-derived challenge (Montgomery limbs) for g given as [npts][L] Montgomery limbs."""
         f = get_field(bit_width)
@@ -255,6 +261,14 @@ class Proof:
         d, pr, nb = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_uint64()
         _lib.check(self.lib.sc_proof_info(self.handle, ctypes.byref(d), ctypes.byref(pr), ctypes.byref(nb)))
         return {"degree": d.value, "pass_rounds": pr.value, "device_bytes": nb.value}
+
+    def use_compiled(self, enable: bool = True) -> int:
+        """Use the compiled kernels of a matching composition (default) or force the
+        interpreted ones. Returns the matching compiled composition id (0: none)."""
+        rc = self.lib.sc_proof_use_compiled(self.handle, 1 if enable else 0)
+        if rc < 0:
+            _lib.check(-rc)
+        return rc
 
     def set_profiling(self, enable: bool = True) -> None:
         _lib.check(self.lib.sc_proof_set_profiling(self.handle, 1 if enable else 0))

@@ -196,6 +196,25 @@ def test_package_api_matches_oracle_reference_contract(bw):
         assert evals.tolist() == ref_evals
 
 
+@pytest.mark.parametrize("bw", WIDTHS)
+def test_compiled_and_interpreted_compositions_agree(ctx, bw):
+    """a*b, a*b*c and a*b*c + d*e have compiled kernels; forcing the interpreted ones gives
+    the same bits."""
+    for name, sid in (("zk_spartan_2", 1), ("poly_abc", 2), ("poly_abc_plus_de", 3)):
+        comp = Composition.from_spec(get_spec(name))
+        with Proof(ctx, bw, comp, 13, input="uniform", seed=17) as pr:
+            assert pr.use_compiled(True) == sid
+            pr.run("sha3")
+            pr.wait()
+            compiled = pr.result()
+            pr.use_compiled(False)
+            pr.run("sha3")
+            pr.wait()
+            interp = pr.result()
+        say("compiled-vs-interpreted", name, bw)
+        assert np.array_equal(compiled[0], interp[0]) and np.array_equal(compiled[1], interp[1])
+
+
 @pytest.mark.parametrize("bw", [64, 256])
 def test_schedule_and_graph_replays_are_identical(ctx, bw):
     comp = Composition.from_spec(get_spec("poly_abc_plus_de"))
