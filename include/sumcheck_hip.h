@@ -82,8 +82,11 @@ int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* co
                     int32_t num_vars, const sc_input* input, sc_proof** out);
 
 /* Tunables of the round schedule: pass grid cap and the pair count at or below which the
- * remaining rounds run in the single-workgroup tail kernel. 0 keeps the default. */
-int sc_proof_set_schedule(sc_proof* proof, int32_t max_pass_blocks, int64_t tail_pairs);
+ * remaining rounds run in the single-workgroup tail kernel (0 keeps the default), and
+ * fused_finalize = 1 to let each pass kernel's last block finish the round instead of a
+ * separate finalize launch. */
+int sc_proof_set_schedule(sc_proof* proof, int32_t max_pass_blocks, int64_t tail_pairs,
+                          int32_t fused_finalize);
 
 /* Enqueues one whole proof (all rounds, transcript on the device) on the context's stream
  * and returns without waiting: the reference's run_once (field_sumcheck.cu:1012-1088).

@@ -101,7 +101,7 @@ SC_HD uint32_t sub_p(Elem<F::L>& out, const Elem<F::L>& a) {
 }
 
 template <class F>
-SC_HD Elem<F::L> add(const Elem<F::L>& a, const Elem<F::L>& b) {
+SC_HD Elem<F::L> add_limbs(const Elem<F::L>& a, const Elem<F::L>& b) {
   Elem<F::L> s, t;
   uint32_t carry = 0;
 #pragma unroll
@@ -119,7 +119,7 @@ SC_HD Elem<F::L> add(const Elem<F::L>& a, const Elem<F::L>& b) {
 }
 
 template <class F>
-SC_HD Elem<F::L> sub(const Elem<F::L>& a, const Elem<F::L>& b) {
+SC_HD Elem<F::L> sub_limbs(const Elem<F::L>& a, const Elem<F::L>& b) {
   Elem<F::L> d, t;
   uint32_t borrow = 0;
 #pragma unroll
@@ -183,6 +183,115 @@ SC_HD Elem<F::L> mont_mul_cios(const Elem<F::L>& a, const Elem<F::L>& b) {
   for (int i = 0; i < L; ++i) r.v[i] = use_q ? q.v[i] : r.v[i];
   return r;
 }
+
+// ---------------------------------------------------------------------------------------------
+// Goldilocks on 64-bit words. p = 2^64 - 2^32 + 1, so 2^64 = 2^32 - 1 (mod p): a carry out of
+// bit 63 is folded back by adding 2^32 - 1, a borrow by subtracting it.
+// ---------------------------------------------------------------------------------------------
+constexpr uint64_t GL_P = 0xFFFFFFFF00000001ull;
+constexpr uint64_t GL_EPS = 0xFFFFFFFFull;  // 2^64 mod p
+
+SC_HD uint64_t gl_u64(const Elem<2>& a) { return (uint64_t)a.v[0] | ((uint64_t)a.v[1] << 32); }
+SC_HD Elem<2> gl_elem(uint64_t x) {
+  Elem<2> e;
+  e.v[0] = (uint32_t)x;
+  e.v[1] = (uint32_t)(x >> 32);
+  return e;
+}
+// x in [0, 2^64) -> [0, p) (2^64 < 2p)
+SC_HD uint64_t gl_canon(uint64_t x) { return x >= GL_P ? x - GL_P : x; }
+// a, b < p -> (a + b) mod p
+SC_HD uint64_t gl_add(uint64_t a, uint64_t b) {
+  const uint64_t s = a + b;
+  const uint64_t t = s + GL_EPS;  // s - p mod 2^64
+  return (s < a || s >= GL_P) ? t : s;
+}
+// a, b < p -> (a - b) mod p
+SC_HD uint64_t gl_sub(uint64_t a, uint64_t b) {
+  const uint64_t d = a - b;
+  return a < b ? d - GL_EPS : d;  // d + p mod 2^64
+}
+// One operand < p, the other < 2^64: a value < 2^64 congruent to a + b (not reduced).
+SC_HD uint64_t gl_add_lazy(uint64_t a, uint64_t b) {
+  const uint64_t s = a + b;
+  return s < a ? s + GL_EPS : s;
+}
+
+// Canonical add/sub for every field (limb chains; 64-bit words for Goldilocks).
+template <class F>
+SC_HD Elem<F::L> add(const Elem<F::L>& a, const Elem<F::L>& b) {
+  if constexpr (F::WIDTH == 64) {
+    return gl_elem(gl_add(gl_u64(a), gl_u64(b)));
+  } else {
+    return add_limbs<F>(a, b);
+  }
+}
+
+template <class F>
+SC_HD Elem<F::L> sub(const Elem<F::L>& a, const Elem<F::L>& b) {
+  if constexpr (F::WIDTH == 64) {
+    return gl_elem(gl_sub(gl_u64(a), gl_u64(b)));
+  } else {
+    return sub_limbs<F>(a, b);
+  }
+}
+
+// The interpolation step x(t+1) = x(t) + d with d < p. For Goldilocks x(t) may be left
+// unreduced in [0, 2^64) (the Montgomery product accepts any 64-bit operands); other fields
+// reduce.
+template <class F>
+SC_HD Elem<F::L> step_add(const Elem<F::L>& x, const Elem<F::L>& d) {
+  if constexpr (F::WIDTH == 64) {
+    return gl_elem(gl_add_lazy(gl_u64(x), gl_u64(d)));
+  } else {
+    return add<F>(x, d);
+  }
+}
+
+// x -> canonical form (identity except for unreduced Goldilocks values).
+template <class F>
+SC_HD Elem<F::L> canon(const Elem<F::L>& x) {
+  if constexpr (F::WIDTH == 64) {
+    return gl_elem(gl_canon(gl_u64(x)));
+  } else {
+    return x;
+  }
+}
+
+// Running sum of field elements. Generic fields add modulo p each time; Goldilocks keeps a
+// 64-bit sum plus a count of carries out of bit 63 (each worth 2^32 - 1) and reduces once.
+template <class F>
+struct Accum {
+  Elem<F::L> v;
+  SC_HD void clear() { v = zero<F>(); }
+  SC_HD void add(const Elem<F::L>& x) { v = sc::add<F>(v, canon<F>(x)); }
+  SC_HD void sub(const Elem<F::L>& x) { v = sc::sub<F>(v, canon<F>(x)); }
+  SC_HD Elem<F::L> value() const { return v; }
+};
+
+template <>
+struct Accum<FieldP64> {
+  uint64_t lo;
+  uint32_t carries;
+  SC_HD void clear() {
+    lo = 0;
+    carries = 0;
+  }
+  SC_HD void add(const Elem<2>& x) {  // any x < 2^64
+    const uint64_t v = gl_u64(x);
+    lo += v;
+    carries += lo < v ? 1u : 0u;
+  }
+  SC_HD void sub(const Elem<2>& x) {  // adds p - canon(x) (0 stays 0)
+    const uint64_t c = gl_canon(gl_u64(x));
+    add(gl_elem(c == 0 ? 0 : GL_P - c));
+  }
+  SC_HD Elem<2> value() const {
+    // lo + carries * (2^32 - 1), carries < 2^32
+    const uint64_t k = (uint64_t)carries * GL_EPS;
+    return gl_elem(gl_canon(gl_add_lazy(gl_canon(lo), k)));
+  }
+};
 
 // acc (64-bit) + top (32-bit) += a * b. On the device this is v_mad_u64_u32 with its carry
 // out, then v_addc_co_u32 into top: two instructions per multiply-accumulate, with the

@@ -2,7 +2,8 @@
 // inst_f*.hip). Reference semantics: cuda/src/field_sumcheck.cu:593-694 (fill, eval, reduce,
 // fold) and :1012-1088 (round loop); src/zkduel/reference.py:78-108.
 //
-// One round j is one `pass_kernel` launch plus one single-block `finalize_kernel` launch:
+// One round j is one `pass_kernel` launch plus one single-block `finalize_kernel` launch (or,
+// with fused finalize, the pass kernel's last block to finish does the finalize work):
 //   pass_kernel     each thread owns pairs i of round j. For j >= 1 it reads entries
 //                   4i..4i+3 of round j-1's tables, folds them with r_{j-1} into round j's
 //                   entries 2i, 2i+1 (written back for round j+1), and evaluates the
@@ -83,7 +84,7 @@ __device__ __forceinline__ Elem<L> load_elem(const uint32_t* p) {
 // C(x) = sum_k c_k prod_v x_v^e_kv. Term structure is wave-uniform, so every branch below is
 // a scalar branch; table slots are indexed statically (unrolled v), keeping x in VGPRs.
 template <class F, int TMAX>
-__device__ __forceinline__ void compose_add(Elem<F::L>& acc, const Elem<F::L> (&x)[TMAX],
+__device__ __forceinline__ void compose_add(Accum<F>& acc, const Elem<F::L> (&x)[TMAX],
                                             const CompDesc* __restrict__ cd, int K) {
   constexpr int L = F::L;
   for (int k = 0; k < K; ++k) {
@@ -103,11 +104,11 @@ __device__ __forceinline__ void compose_add(Elem<F::L>& acc, const Elem<F::L> (&
     }
     const int kind = cd->kind[k];
     if (kind == COEFF_ONE) {
-      acc = add<F>(acc, prod);
+      acc.add(prod);
     } else if (kind == COEFF_MINUS_ONE) {
-      acc = sub<F>(acc, prod);
+      acc.sub(prod);
     } else {
-      acc = add<F>(acc, mont_mul<F>(prod, load_elem<L>(cd->coeff[k])));
+      acc.add(mont_mul<F>(prod, load_elem<L>(cd->coeff[k])));
     }
   }
 }
@@ -115,7 +116,7 @@ __device__ __forceinline__ void compose_add(Elem<F::L>& acc, const Elem<F::L> (&
 // Compiled composition STATIC_COMPS[SID]: every loop has constant bounds and indices, so it
 // unrolls into straight-line field operations on fixed registers.
 template <class F, int SID, int TMAX>
-__device__ __forceinline__ void compose_static(Elem<F::L>& acc, const Elem<F::L> (&x)[TMAX],
+__device__ __forceinline__ void compose_static(Accum<F>& acc, const Elem<F::L> (&x)[TMAX],
                                                const CompDesc* __restrict__ cd) {
   constexpr StaticComp C = STATIC_COMPS[SID];
   static_assert(C.num_tables <= TMAX, "table slots");
@@ -125,17 +126,17 @@ __device__ __forceinline__ void compose_static(Elem<F::L>& acc, const Elem<F::L>
 #pragma unroll
     for (int m = 1; m < C.lens[k]; ++m) prod = mont_mul<F>(prod, x[C.vars[k][m]]);
     if (C.kinds[k] == COEFF_ONE) {
-      acc = add<F>(acc, prod);
+      acc.add(prod);
     } else if (C.kinds[k] == COEFF_MINUS_ONE) {
-      acc = sub<F>(acc, prod);
+      acc.sub(prod);
     } else {
-      acc = add<F>(acc, mont_mul<F>(prod, load_elem<F::L>(cd->coeff[k])));
+      acc.add(mont_mul<F>(prod, load_elem<F::L>(cd->coeff[k])));
     }
   }
 }
 
 template <class F, int SID, int TMAX>
-__device__ __forceinline__ void compose(Elem<F::L>& acc, const Elem<F::L> (&x)[TMAX],
+__device__ __forceinline__ void compose(Accum<F>& acc, const Elem<F::L> (&x)[TMAX],
                                         const CompDesc* __restrict__ cd, int K) {
   if constexpr (SID == 0) {
     compose_add<F, TMAX>(acc, x, cd, K);
@@ -160,10 +161,11 @@ struct Shape {
 
 // cur holds f[2i] and oth holds f[2i+1] of every table on entry. Accumulates
 // C(f[2i] + t (f[2i+1] - f[2i])) into acc[t] for t < npts, with t >= 2 reached by repeated
-// addition of the difference (reference.py:89-99).
+// addition of the difference (reference.py:89-99); those sums may stay unreduced where the
+// field allows (step_add).
 template <class F, int TMAX, int PMAX, int SID>
 __device__ __forceinline__ void eval_points(Elem<F::L> (&cur)[TMAX], Elem<F::L> (&oth)[TMAX],
-                                            Elem<F::L> (&acc)[PMAX], const CompDesc* __restrict__ cd,
+                                            Accum<F> (&acc)[PMAX], const CompDesc* __restrict__ cd,
                                             int T, int K, int npts) {
   compose<F, SID, TMAX>(acc[0], cur, cd, K);
 #pragma unroll
@@ -180,7 +182,7 @@ __device__ __forceinline__ void eval_points(Elem<F::L> (&cur)[TMAX], Elem<F::L> 
     if (t < npts) {
 #pragma unroll
       for (int v = 0; v < TMAX; ++v) {
-        if (v < T) cur[v] = add<F>(cur[v], oth[v]);
+        if (v < T) cur[v] = step_add<F>(cur[v], oth[v]);
       }
       compose<F, SID, TMAX>(acc[t], cur, cd, K);
     }
@@ -336,20 +338,45 @@ __device__ __forceinline__ void finish_round(const uint32_t* g, int npts, int ro
 // Kernels
 // ---------------------------------------------------------------------------------------------
 
+// Sums the per-block partials of a pass and finishes the round (g_j, r_j). Used by the
+// finalize kernel and by the last block of a pass kernel.
+template <class F, int PMAX>
+__device__ __forceinline__ void reduce_partials_and_finish(const uint32_t* partials, uint32_t nblocks,
+                                                           int npts, int round, int num_rounds, int mode,
+                                                           const FieldConsts* fc, ProofState* st,
+                                                           const uint32_t* explicit_r, uint32_t* s_red,
+                                                           uint32_t* s_g, uint32_t* s_msg, uint32_t* s_r) {
+  constexpr int L = F::L;
+  Elem<L> acc[PMAX];
+#pragma unroll
+  for (int t = 0; t < PMAX; ++t) acc[t] = zero<F>();
+  for (uint32_t b = threadIdx.x; b < nblocks; b += BLOCK) {
+#pragma unroll
+    for (int t = 0; t < PMAX; ++t) {
+      if (t < npts) acc[t] = add<F>(acc[t], load_elem<L>(partials + ((uint64_t)b * MAX_POINTS + t) * L));
+    }
+  }
+  block_reduce<F, PMAX, BLOCK>(acc, npts, s_red, s_g);
+  finish_round<F, BLOCK>(s_g, npts, round, num_rounds, mode, fc, st, explicit_r, s_msg, s_r);
+}
+
 template <class F, int TMAX, int PMAX, int SID>
 __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
   constexpr int L = F::L;
   __shared__ uint32_t s_red[(BLOCK / 64) * PMAX * L];
   __shared__ uint32_t s_g[PMAX * L];
+  __shared__ uint32_t s_msg[3 * 34];
+  __shared__ uint32_t s_r[L];
+  __shared__ int s_last;
   const CompDesc* __restrict__ cd = a.comp;
   const int T = Shape<SID>::tables(cd);
   const int K = Shape<SID>::terms(cd);
   const int npts = Shape<SID>::npts(cd);
   const bool fold = a.fold != 0;
 
-  Elem<L> acc[PMAX];
+  Accum<F> acc[PMAX];
 #pragma unroll
-  for (int t = 0; t < PMAX; ++t) acc[t] = zero<F>();
+  for (int t = 0; t < PMAX; ++t) acc[t].clear();
   Elem<L> r = zero<F>();
   if (fold) r = load_elem<L>(a.st->r[a.round - 1]);
 
@@ -359,13 +386,29 @@ __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
     load_pair<F, TMAX>(cur, oth, a.src, a.dst, a.src_n, i, T, fold, r);
     eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts);
   }
-  block_reduce<F, PMAX, BLOCK>(acc, npts, s_red, s_g);
+  Elem<L> sums[PMAX];
+#pragma unroll
+  for (int t = 0; t < PMAX; ++t) sums[t] = acc[t].value();
+  block_reduce<F, PMAX, BLOCK>(sums, npts, s_red, s_g);
   if (threadIdx.x < npts * L) {
     a.partials[(uint64_t)blockIdx.x * (MAX_POINTS * L) + threadIdx.x] = s_g[threadIdx.x];
+    if (a.fused_finalize) __threadfence();  // release this block's partial device-wide
   }
+  if (!a.fused_finalize) return;
+  // Last-block-done: the block that increments the counter last sees every partial.
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned prev = atomicAdd(&a.st->counter, 1u);
+    s_last = prev == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();  // acquire the other blocks' partials
+  reduce_partials_and_finish<F, PMAX>(a.partials, gridDim.x, npts, a.round, a.num_rounds, a.challenge_mode,
+                                      a.fc, a.st, a.explicit_r, s_red, s_g, s_msg, s_r);
+  if (threadIdx.x == 0) a.st->counter = 0u;  // ready for the next pass
 }
-
-
 
 template <class F, int PMAX>
 __global__ void __launch_bounds__(BLOCK) finalize_kernel(FinalizeArgs a) {
@@ -374,18 +417,8 @@ __global__ void __launch_bounds__(BLOCK) finalize_kernel(FinalizeArgs a) {
   __shared__ uint32_t s_g[PMAX * L];
   __shared__ uint32_t s_msg[3 * 34];
   __shared__ uint32_t s_r[L];
-  const int npts = a.comp->npts;
-  Elem<L> acc[PMAX];
-#pragma unroll
-  for (int t = 0; t < PMAX; ++t) acc[t] = zero<F>();
-  for (uint32_t b = threadIdx.x; b < a.nblocks; b += BLOCK) {
-#pragma unroll
-    for (int t = 0; t < PMAX; ++t) {
-      if (t < npts) acc[t] = add<F>(acc[t], load_elem<L>(a.partials + ((uint64_t)b * MAX_POINTS + t) * L));
-    }
-  }
-  block_reduce<F, PMAX, BLOCK>(acc, npts, s_red, s_g);
-  finish_round<F, BLOCK>(s_g, npts, a.round, a.num_rounds, a.challenge_mode, a.fc, a.st, a.explicit_r, s_msg, s_r);
+  reduce_partials_and_finish<F, PMAX>(a.partials, a.nblocks, a.comp->npts, a.round, a.num_rounds,
+                                      a.challenge_mode, a.fc, a.st, a.explicit_r, s_red, s_g, s_msg, s_r);
 }
 
 // One workgroup of TB threads (more than a pass block: the tail's rounds have no other
@@ -411,15 +444,18 @@ __global__ void __launch_bounds__(TB) tail_kernel(TailArgs a) {
     const uint64_t src_n = fold ? (a.n0 >> (j - 1)) : a.n0;
     const uint64_t pairs = a.n0 >> (j + 1);
     const Elem<L> r = load_elem<L>(s_r);
-    Elem<L> acc[PMAX];
+    Accum<F> acc[PMAX];
 #pragma unroll
-    for (int t = 0; t < PMAX; ++t) acc[t] = zero<F>();
+    for (int t = 0; t < PMAX; ++t) acc[t].clear();
     for (uint64_t i = threadIdx.x; i < pairs; i += TB) {
       Elem<L> cur[TMAX], oth[TMAX];
       load_pair<F, TMAX>(cur, oth, src, dst, src_n, i, T, fold, r);
       eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts);
     }
-    block_reduce<F, PMAX, TB>(acc, npts, s_red, s_g);
+    Elem<L> sums[PMAX];
+#pragma unroll
+    for (int t = 0; t < PMAX; ++t) sums[t] = acc[t].value();
+    block_reduce<F, PMAX, TB>(sums, npts, s_red, s_g);
     finish_round<F, TB>(s_g, npts, j, a.num_rounds, a.challenge_mode, a.fc, a.st, a.explicit_r, s_msg, s_r);
     __syncthreads();
   }

@@ -84,6 +84,8 @@ struct PassArgs {
   int32_t num_rounds;
   int32_t fold;  // 0 for round 0 (pairs read straight from src), 1 otherwise
   int32_t challenge_mode;
+  int32_t fused_finalize;  // the last block to finish reduces the partials and derives r
+  int32_t pad0;
   const CompDesc* comp;
   const FieldConsts* fc;
   ProofState* st;

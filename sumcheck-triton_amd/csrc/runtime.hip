@@ -217,6 +217,7 @@ struct sc_proof {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int max_pass_blocks = 2048;
   int64_t tail_pairs = 0;
+  bool fused_finalize = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_mode = -1;
@@ -269,11 +270,14 @@ int enqueue_rounds(sc_proof* p, int mode) {
     a.st = p->d_state;
     a.partials = p->d_partials;
     a.explicit_r = p->d_explicit;
+    a.fused_finalize = p->fused_finalize ? 1 : 0;
+    a.pad0 = 0;
     uint64_t g = (a.pairs + BLOCK - 1) / BLOCK;
     if (g > (uint64_t)p->max_pass_blocks) g = (uint64_t)p->max_pass_blocks;
     if (p->profiling) SC_HIP(hipEventRecord(p->prof_ev[2 * j], s));
     SC_HIP(p->ln->pass(s, p->sid, p->tmax, p->pmax, a, (unsigned)g));
     if (p->profiling) SC_HIP(hipEventRecord(p->prof_ev[2 * j + 1], s));
+    if (p->fused_finalize) continue;
     FinalizeArgs f;
     f.partials = p->d_partials;
     f.nblocks = (uint32_t)g;
@@ -502,8 +506,10 @@ int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* co
   return SC_OK;
 }
 
-int sc_proof_set_schedule(sc_proof* p, int32_t max_pass_blocks, int64_t tail_pairs) {
+int sc_proof_set_schedule(sc_proof* p, int32_t max_pass_blocks, int64_t tail_pairs, int32_t fused_finalize) {
   if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
+  if (fused_finalize < 0 || fused_finalize > 1) return fail(SC_ERR_ARG, "fused_finalize must be 0 or 1");
+  p->fused_finalize = fused_finalize != 0;
   if (max_pass_blocks < 0 || max_pass_blocks > MAX_PASS_BLOCKS)
     return fail(SC_ERR_ARG, "max_pass_blocks must be in [0, 4096]");
   if (tail_pairs < 0) return fail(SC_ERR_ARG, "tail_pairs must be >= 0");

@@ -224,8 +224,9 @@ class Proof:
             return np.zeros((1, self.field.limbs), dtype=np.uint32)
         return self.field.ints_to_limbs(vals)
 
-    def set_schedule(self, max_pass_blocks: int = 0, tail_pairs: int = 0) -> None:
-        _lib.check(self.lib.sc_proof_set_schedule(self.handle, int(max_pass_blocks), int(tail_pairs)))
+    def set_schedule(self, max_pass_blocks: int = 0, tail_pairs: int = 0, fused_finalize: bool = False) -> None:
+        _lib.check(self.lib.sc_proof_set_schedule(self.handle, int(max_pass_blocks), int(tail_pairs),
+                                                  1 if fused_finalize else 0))
 
     def run(self, challenge_mode: str = "sha3", challenges=None) -> None:
         mode = _lib.CHALLENGE_MODES[challenge_mode]

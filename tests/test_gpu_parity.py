@@ -223,9 +223,10 @@ def test_schedule_and_graph_replays_are_identical(ctx, bw):
         pr.run("sha3")
         pr.wait()
         base = pr.result()
-        for blocks, tail in ((16, 1), (4096, 8), (2048, 1 << 20), (0, 0)):
-            say("schedule", bw, blocks, tail)
-            pr.set_schedule(blocks, tail)
+        for blocks, tail, fused in ((16, 1, False), (4096, 8, False), (2048, 1 << 20, False),
+                                    (16, 1, True), (4096, 8, True), (0, 0, True), (0, 0, False)):
+            say("schedule", bw, blocks, tail, fused)
+            pr.set_schedule(blocks, tail, fused)
             pr.run("sha3")
             pr.wait()
             got = pr.result()
