@@ -49,7 +49,15 @@ def pass_bytes(T: int, limbs: int, n0: int, j: int) -> int:
     return T * w * ((n0 >> (j - 1)) + (n0 >> j))
 
 
-def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int) -> dict:
+def modmuls_per_pair(spec, fold: bool) -> int:
+    """Field multiplications per pair of a round: (d+1) evaluations of the composition
+    (each term's len-1 products, plus one for a coefficient other than +-1) and, when the
+    round folds, two per table."""
+    per_point = sum(n - 1 for n in spec.term_lens) + sum(1 for c in spec.coeffs if c not in (1, -1))
+    return (spec.degree + 1) * per_point + (2 * spec.num_vars if fold else 0)
+
+
+def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: float | None) -> dict:
     from zkduel_hip import Composition, Proof, get_field, get_spec
     from zkduel_hip.driver import algorithmic_bytes
 
@@ -63,6 +71,8 @@ def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int) -> dict:
             pr.run("sha3")
             pr.wait()
         lat, pass_ms_sum, pass_bytes_sum, pass_launches, dev_ms = [], 0.0, 0, 0, []
+        pass_macs_sum = 0
+        limb_macs = 2 * f.limbs * f.limbs + f.limbs  # 32x32 multiply-accumulates per product
         ctx.sync()
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -75,6 +85,7 @@ def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int) -> dict:
             for j, ms in enumerate(passes):
                 pass_ms_sum += ms
                 pass_bytes_sum += pass_bytes(T, f.limbs, 1 << nv, j)
+                pass_macs_sum += ((1 << nv) >> (j + 1)) * modmuls_per_pair(spec, j > 0) * limb_macs
                 pass_launches += 1
         ctx.sync()
         wall = time.perf_counter() - t0
@@ -97,6 +108,8 @@ def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int) -> dict:
         "pass_kernel_avg_bytes": pass_bytes_sum / max(pass_launches, 1),
         "pass_rounds": info["pass_rounds"],
         "hbm_frac": achieved / HBM_PEAK_GBPS,
+        "pass_limb_macs_per_s": pass_macs_sum / (pass_ms_sum * 1e-3) if pass_ms_sum > 0 else 0.0,
+        "int32_frac": (pass_macs_sum / (pass_ms_sum * 1e-3)) / mad_rate if (mad_rate and pass_ms_sum > 0) else None,
         "wall_s": wall,
     }
 
@@ -131,6 +144,15 @@ def cpu_baseline(budget_s: float = 20.0) -> dict:
     }
 
 
+def traffic_bytes():
+    """HBM bytes per pass-kernel launch measured by rocprofv3 FETCH_SIZE + WRITE_SIZE (committed
+    under profiles/ with the command that produced it), or None."""
+    path = ROOT / "profiles" / "pass_kernel_traffic_goldilocks_nv26.json"
+    if not path.exists():
+        return None
+    return json.loads(path.read_text()).get("traffic_bytes_per_launch")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,11 +171,13 @@ def main() -> int:
     from zkduel_hip import Context
 
     ctx = Context(0)
+    calib = ctx.calibrate()
+    log("bench: calibration " + json.dumps({k: f"{v:.4g}" for k, v in calib.items()}))
     per = {}
     for bw in args.widths:
         steps = args.steps if bw == 64 else max(3, args.steps // 2)
         log(f"bench: width {bw} num_vars {SIZES[bw]} steps {steps}")
-        per[bw] = measure_width(ctx, bw, SIZES[bw], steps, args.warmup)
+        per[bw] = measure_width(ctx, bw, SIZES[bw], steps, args.warmup, calib["mad_u64_u32"])
         log(f"bench: width {bw}: {per[bw]['proofs_per_s']:.1f} proofs/s, "
             f"pass kernel {per[bw]['pass_kernel_gbps']:.0f} GB/s")
     head = per[64]
@@ -187,6 +211,12 @@ def main() -> int:
                 str(bw): {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
                 for bw, r in per.items()
             },
+            "calibration": {
+                "mad_u64_u32_lane_ops_per_s": calib["mad_u64_u32"],
+                "hbm_read_gbps": calib["hbm_read"] / 1e9,
+                "hbm_copy_gbps": calib["hbm_copy"] / 1e9,
+                "pass_kernel_frac_of_measured_copy": head["pass_kernel_gbps"] / (calib["hbm_copy"] / 1e9),
+            },
         },
         "roofline": {
             "bound": "hbm",
@@ -194,7 +224,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(head["pass_kernel_gbps"] / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": traffic_bytes(),
         },
         "cpu_baseline": cpu,
     }

@@ -71,6 +71,59 @@ __global__ void __launch_bounds__(256) calib_fma_f64(double b, int iters, double
   out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// Streaming HBM probes over `words` uint4 elements: a read-only XOR reduction and a copy.
+__global__ void __launch_bounds__(256) calib_read(const uint4* __restrict__ src, uint64_t words, uint32_t* out) {
+  uint32_t acc = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < words; i += (uint64_t)gridDim.x * 256) {
+    const uint4 v = src[i];
+    acc ^= v.x ^ v.y ^ v.z ^ v.w;
+  }
+  out[blockIdx.x * 256 + threadIdx.x] = acc;
+}
+
+__global__ void __launch_bounds__(256) calib_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t words) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < words; i += (uint64_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+hipError_t calibrate_bandwidth(hipStream_t s, double* rates, int n) {
+  const uint64_t bytes = (uint64_t)1 << 31;  // 2 GiB per buffer: far beyond the 256 MB last-level cache
+  const uint64_t words = bytes / 16;
+  void* src = nullptr;
+  void* dst = nullptr;
+  hipError_t e = hipMalloc(&src, bytes);
+  if (e == hipSuccess) e = hipMalloc(&dst, bytes);
+  if (e == hipSuccess) e = hipMemsetAsync(src, 1, bytes, s);
+  hipEvent_t a = nullptr, b = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&a);
+  if (e == hipSuccess) e = hipEventCreate(&b);
+  const int blocks = 4096;
+  for (int k = 0; k < n && k < 2 && e == hipSuccess; ++k) {
+    float best = 1e30f;
+    for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {
+      e = hipEventRecord(a, s);
+      if (e != hipSuccess) break;
+      if (k == 0) {
+        hipLaunchKernelGGL(calib_read, dim3(blocks), dim3(256), 0, s, (const uint4*)src, words, (uint32_t*)dst);
+      } else {
+        hipLaunchKernelGGL(calib_copy, dim3(blocks), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, words);
+      }
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipEventRecord(b, s);
+      if (e == hipSuccess) e = hipEventSynchronize(b);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+      if (e == hipSuccess && rep > 0 && ms < best) best = ms;
+    }
+    // bytes moved per second: read probe reads `bytes`, copy reads and writes them
+    if (e == hipSuccess) rates[k] = (k == 0 ? 1.0 : 2.0) * (double)bytes / (best * 1e-3);
+  }
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  if (src) (void)hipFree(src);
+  if (dst) (void)hipFree(dst);
+  return e;
+}
+
 hipError_t calibrate(hipStream_t s, double* rates, int n) {
   // grid: 8 blocks per CU of a 256-CU device, 256 threads each
   const int blocks = 2048, threads = 256, iters = 4096;
@@ -80,7 +133,7 @@ hipError_t calibrate(hipStream_t s, double* rates, int n) {
   hipEvent_t a = nullptr, b = nullptr;
   e = hipEventCreate(&a);
   if (e == hipSuccess) e = hipEventCreate(&b);
-  for (int k = 0; k < n && k < CALIB_COUNT && e == hipSuccess; ++k) {
+  for (int k = 0; k < n && k < CALIB_HBM_READ && e == hipSuccess; ++k) {
     for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {  // rep 0 warms up
       e = hipEventRecord(a, s);
       if (e != hipSuccess) break;
@@ -111,6 +164,7 @@ hipError_t calibrate(hipStream_t s, double* rates, int n) {
   if (a) (void)hipEventDestroy(a);
   if (b) (void)hipEventDestroy(b);
   (void)hipFree(buf);
+  if (e == hipSuccess && n > CALIB_HBM_READ) e = calibrate_bandwidth(s, rates + CALIB_HBM_READ, n - CALIB_HBM_READ);
   return e;
 }
 

@@ -5,9 +5,18 @@
 
 namespace sc {
 
-enum CalibOp { CALIB_MAD_U64 = 0, CALIB_MUL_HI_U32 = 1, CALIB_ADD_CO = 2, CALIB_FMA_F64 = 3, CALIB_COUNT = 4 };
+enum CalibOp {
+  CALIB_MAD_U64 = 0,
+  CALIB_MUL_HI_U32 = 1,
+  CALIB_ADD_CO = 2,
+  CALIB_FMA_F64 = 3,
+  CALIB_HBM_READ = 4,  // bytes/s, streaming read of 2 GiB
+  CALIB_HBM_COPY = 5,  // bytes/s read + written, streaming copy of 2 GiB
+  CALIB_COUNT = 6
+};
 
-// rates[k] = sustained lane operations per second of op k (n <= CALIB_COUNT entries).
+// rates[k] = sustained lane operations per second of op k (k < 4) or HBM bytes per second
+// (k = 4, 5); n <= CALIB_COUNT entries.
 hipError_t calibrate(hipStream_t s, double* rates, int n);
 
 }  // namespace sc

@@ -229,14 +229,19 @@ struct sc_proof {
 
 namespace {
 
-int64_t default_tail_pairs(int L) {
-  // One workgroup handles the last rounds; keep its work small next to a pass launch.
+// Schedule defaults per field, from sweeps on MI355X (a*b*c + d*e at num_vars 24-26): the
+// pass grid cap, and the pair count at which the remaining rounds move to the tail kernel.
+int default_pass_blocks(int L) {
   switch (L) {
     case 1: return 4096;
-    case 2: return 2048;
-    case 4: return 512;
-    default: return 128;
+    case 2: return 1024;
+    case 4: return 2048;
+    default: return 1024;
   }
+}
+
+int64_t default_tail_pairs(int L) {
+  return L == 8 ? 64 : 512;
 }
 
 // First round that runs in the tail kernel.
@@ -439,6 +444,7 @@ int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* co
   p->pmax = p->npts <= 4 ? 4 : 8;
   p->n0 = (uint64_t)1 << num_vars;
   p->tail_pairs = default_tail_pairs(p->L);
+  p->max_pass_blocks = default_pass_blocks(p->L);
   p->matched_sid = match_static(desc);
   p->sid = p->matched_sid;
 
@@ -513,7 +519,7 @@ int sc_proof_set_schedule(sc_proof* p, int32_t max_pass_blocks, int64_t tail_pai
   if (max_pass_blocks < 0 || max_pass_blocks > MAX_PASS_BLOCKS)
     return fail(SC_ERR_ARG, "max_pass_blocks must be in [0, 4096]");
   if (tail_pairs < 0) return fail(SC_ERR_ARG, "tail_pairs must be >= 0");
-  p->max_pass_blocks = max_pass_blocks ? max_pass_blocks : 2048;
+  p->max_pass_blocks = max_pass_blocks ? max_pass_blocks : default_pass_blocks(p->L);
   p->tail_pairs = tail_pairs ? tail_pairs : default_tail_pairs(p->L);
   if (p->graph_exec) {
     (void)hipGraphExecDestroy(p->graph_exec);
@@ -744,7 +750,7 @@ int sc_transcript_challenge(sc_context* ctx, int32_t bit_width, int32_t round, c
 
 int sc_calibrate(sc_context* ctx, double* rates, int32_t n) {
   if (ctx == nullptr || rates == nullptr) return fail(SC_ERR_ARG, "NULL argument");
-  if (n < 0 || n > CALIB_COUNT) return fail(SC_ERR_ARG, "n must be in [0, 4]");
+  if (n < 0 || n > CALIB_COUNT) return fail(SC_ERR_ARG, "n must be in [0, 6]");
   SC_HIP(hipSetDevice(ctx->device));
   for (int k = 0; k < n; ++k) rates[k] = 0.0;
   SC_HIP(calibrate(ctx->stream, rates, n));

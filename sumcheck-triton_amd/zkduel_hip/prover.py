@@ -127,10 +127,12 @@ class Context:
         return out
 
     def calibrate(self) -> dict[str, float]:
-        """Measured lane rates (ops/s): mad_u64_u32, mul_hi_u32, add64 (add_co+addc), fma_f64."""
-        rates = (ctypes.c_double * 4)()
-        _lib.check(self.lib.sc_calibrate(self.handle, rates, 4))
-        return {"mad_u64_u32": rates[0], "mul_hi_u32": rates[1], "add64": rates[2], "fma_f64": rates[3]}
+        """Measured lane rates (ops/s) of mad_u64_u32, mul_hi_u32, 64-bit add and fma_f64, and
+        streaming HBM read / copy bandwidth (bytes/s)."""
+        rates = (ctypes.c_double * 6)()
+        _lib.check(self.lib.sc_calibrate(self.handle, rates, 6))
+        keys = ("mad_u64_u32", "mul_hi_u32", "add64", "fma_f64", "hbm_read", "hbm_copy")
+        return dict(zip(keys, list(rates)))
 
     def transcript_challenge(self, bit_width: int, round_idx: int, g_mont: np.ndarray) -> np.ndarray:
         """Device-derived challenge (Montgomery limbs) for g given as [npts][L] Montgomery limbs."""

@@ -1,0 +1,61 @@
+"""Summarize a rocprofv3 results database (``--kernel-trace`` and/or ``--pmc``) as text.
+
+    python -m zkduel_hip.rocprof_summary <results.db> [--pmc COUNTER] [--match SUBSTRING]
+
+Prints per-kernel launch counts and durations (avg/min/max/total), and, with --pmc, the
+counter's average value per dispatch of each kernel. Counters such as FETCH_SIZE and
+WRITE_SIZE are in KiB per dispatch; dividing a kernel's value by the same counter's value
+for a copy of known size (the library's calibration copy, ``calib_copy``, moves 2 GiB read
+and 2 GiB written) turns them into bytes with the probe's own scale.
+"""
+
+from __future__ import annotations
+
+import argparse
+import sqlite3
+from collections import defaultdict
+
+
+def kernel_stats(db: sqlite3.Connection, match: str | None = None) -> list[tuple]:
+    rows = db.execute(
+        "select name, count(*), avg(end-start), min(end-start), max(end-start), sum(end-start) "
+        "from kernels group by name order by sum(end-start) desc"
+    ).fetchall()
+    return [r for r in rows if match is None or match in r[0]]
+
+
+def pmc_per_kernel(db: sqlite3.Connection, counter: str) -> dict[str, tuple[int, float]]:
+    cols = [r[1] for r in db.execute("pragma table_info(pmc_events)")]
+    name_col = "name" if "name" in cols else None
+    counter_col = "counter_name" if "counter_name" in cols else ("pmc_name" if "pmc_name" in cols else None)
+    value_col = "counter_value" if "counter_value" in cols else ("value" if "value" in cols else None)
+    if not (name_col and counter_col and value_col):
+        raise SystemExit(f"unexpected pmc_events columns: {cols}")
+    acc: dict[str, list[float]] = defaultdict(list)
+    for name, val in db.execute(
+        f"select {name_col}, {value_col} from pmc_events where {counter_col} = ?", (counter,)
+    ):
+        acc[name].append(float(val))
+    return {k: (len(v), sum(v) / len(v)) for k, v in acc.items()}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("db")
+    ap.add_argument("--pmc", action="append", default=[])
+    ap.add_argument("--match", default=None)
+    args = ap.parse_args()
+    db = sqlite3.connect(args.db)
+    print(f"{'kernel':70s} {'n':>5s} {'avg_us':>10s} {'min_us':>10s} {'max_us':>10s} {'total_ms':>10s}")
+    for name, n, avg, mn, mx, tot in kernel_stats(db, args.match):
+        print(f"{name[:70]:70s} {n:5d} {avg / 1e3:10.2f} {mn / 1e3:10.2f} {mx / 1e3:10.2f} {tot / 1e6:10.3f}")
+    for counter in args.pmc:
+        print(f"\n{counter} per dispatch (average over dispatches)")
+        for name, (n, avg) in sorted(pmc_per_kernel(db, counter).items(), key=lambda kv: -kv[1][1]):
+            if args.match is None or args.match in name or "calib" in name:
+                print(f"{name[:70]:70s} {n:5d} {avg:16.1f}")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
