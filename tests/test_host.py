@@ -136,3 +136,27 @@ def test_context_without_device_fails_loudly():
 
     with pytest.raises((ValueError, RuntimeError)):
         Context(0)
+
+
+def test_cli_flag_grammar_matches_reference_executable():
+    from zkduel_hip.cli import parse_args
+
+    a = parse_args(["--rounds", "3", "10", "--bit-widths", "64", "256", "--workloads", "zkphire_8",
+                    "poly_abc", "--warmups", "0", "--repeats", "1", "--challenge-mode", "sha3",
+                    "--point-mode", "specialized", "--out", "x.csv", "--zk-only"])
+    assert a["rounds"] == [3, 10] and a["bit_widths"] == [64, 256]
+    assert a["workloads"] == ["zkphire_8", "poly_abc"] and a["zk_only"]
+    assert (a["warmups"], a["repeats"], a["challenge_mode"], a["point_mode"]) == (0, 1, "sha3", "specialized")
+    for bad in (["--block", "3"], ["--timing-mode", "x"], ["--nope"], ["--layout", "limb"], ["--seed"]):
+        with pytest.raises(SystemExit) as e:
+            parse_args(bad)
+        assert e.value.code == 2
+
+
+def test_driver_columns_are_the_reference_csv_header():
+    from zkduel_hip.driver import CSV_COLUMNS
+
+    header = ("workload,polynomial,backend,field,arithmetic,layout,point_mode,bit_width,limbs,modulus_hex,"
+              "rounds,N,vars,degree,terms,warmups,repeats,timing_mode,challenge_mode,first_ms,median_ms,"
+              "eval_median_ms,challenge_median_ms,fold_median_ms,total_median_ms,input_mib,checksum,status")
+    assert ",".join(CSV_COLUMNS) == header  # cuda/src/field_sumcheck.cu:1336
