@@ -143,6 +143,25 @@ int sc_field_op(sc_context* ctx, int32_t bit_width, int32_t op, uint64_t n, cons
 int sc_transcript_challenge(sc_context* ctx, int32_t bit_width, int32_t round, const uint32_t* g_mont,
                             int32_t npts, uint32_t* r_mont);
 
+/* Batched proving (no counterpart in the reference, which proves one instance per run):
+ * n independent instances of one width, each with its own composition, num_vars and
+ * generated input (SC_INPUT_AFFINE or SC_INPUT_UNIFORM), advance round by round; each stage
+ * is one launch over all active instances' tiles plus one launch finishing their rounds.
+ * Results per instance are the same bits as sc_prove on that instance. */
+typedef struct sc_batch sc_batch;
+int sc_batch_create(sc_context* ctx, int32_t bit_width, int32_t n, const sc_composition* comps,
+                    const int32_t* num_vars, const sc_input* inputs, sc_batch** out);
+/* Pairs per thread in a tile (tile = 256 * items pairs); default 4. */
+int sc_batch_set_tile(sc_batch* batch, int32_t items_per_thread);
+/* challenge_mode SC_CHALLENGE_FIXED or SC_CHALLENGE_SHA3; enqueues without waiting. */
+int sc_batch_run(sc_batch* batch, int32_t challenge_mode);
+int sc_batch_capture(sc_batch* batch, int32_t challenge_mode);
+int sc_batch_wait(sc_batch* batch, float* ms);
+int sc_batch_result(sc_batch* batch, int32_t instance, uint32_t* round_polys, uint32_t* challenges,
+                    uint64_t* checksum);
+int sc_batch_info(sc_batch* batch, uint64_t* device_bytes, int32_t* stages, uint64_t* tiles);
+int sc_batch_destroy(sc_batch* batch);
+
 /* Measured sustained lane rates (operations per second) of the VALU operations the field
  * arithmetic is built from: rates[0] v_mad_u64_u32 (32x32+64), rates[1] v_mul_hi_u32,
  * rates[2] 64-bit add, rates[3] v_fma_f64; and streaming HBM bandwidth in bytes/s:

@@ -461,6 +461,91 @@ __global__ void __launch_bounds__(TB) tail_kernel(TailArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Batched stages
+// ---------------------------------------------------------------------------------------------
+
+// Entry of block b: the last entry whose tile_begin <= b (entries sorted by tile_begin).
+__device__ __forceinline__ uint32_t find_entry(const StageEntry* __restrict__ e, uint32_t n, uint32_t b) {
+  uint32_t lo = 0, hi = n;  // invariant: e[lo].tile_begin <= b < e[hi].tile_begin
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (e[mid].tile_begin <= b) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// One stage: block b runs tile (b - tile_begin) of its entry's round, exactly like a
+// pass_kernel block (fold with r_{round-1}, evaluate, reduce), writing one partial per tile.
+template <class F, int TMAX, int PMAX>
+__global__ void __launch_bounds__(BLOCK) batch_pass_kernel(BatchArgs a) {
+  constexpr int L = F::L;
+  __shared__ uint32_t s_red[(BLOCK / 64) * PMAX * L];
+  __shared__ uint32_t s_g[PMAX * L];
+  const uint32_t k = find_entry(a.entries, a.n_entries, blockIdx.x);
+  const StageEntry ent = a.entries[k];
+  const InstDesc& in = a.insts[ent.inst];
+  const CompDesc* __restrict__ cd = in.comp;
+  const int T = cd->num_tables;
+  const int K = cd->num_terms;
+  const int npts = cd->npts;
+  const int j = (int)ent.round;
+  const bool fold = j > 0;
+  const uint32_t* src = j <= 1 ? in.input : (((j - 1) & 1) ? in.buf_a : in.buf_b);
+  uint32_t* dst = (j & 1) ? in.buf_a : in.buf_b;
+  const uint64_t src_n = fold ? (in.n0 >> (j - 1)) : in.n0;
+  const uint64_t pairs = in.n0 >> (j + 1);
+  Elem<L> r = zero<F>();
+  if (fold) r = load_elem<L>(in.st->r[j - 1]);
+
+  Accum<F> acc[PMAX];
+#pragma unroll
+  for (int t = 0; t < PMAX; ++t) acc[t].clear();
+  const uint64_t tile = blockIdx.x - ent.tile_begin;
+  const uint64_t base = tile * (uint64_t)BLOCK * a.items;
+  for (uint32_t it = 0; it < a.items; ++it) {
+    const uint64_t i = base + (uint64_t)it * BLOCK + threadIdx.x;
+    if (i < pairs) {
+      Elem<L> cur[TMAX], oth[TMAX];
+      load_pair<F, TMAX>(cur, oth, src, dst, src_n, i, T, fold, r);
+      eval_points<F, TMAX, PMAX, 0>(cur, oth, acc, cd, T, K, npts);
+    }
+  }
+  Elem<L> sums[PMAX];
+#pragma unroll
+  for (int t = 0; t < PMAX; ++t) sums[t] = acc[t].value();
+  block_reduce<F, PMAX, BLOCK>(sums, npts, s_red, s_g);
+  if (threadIdx.x < npts * L) {
+    a.partials[(uint64_t)blockIdx.x * (MAX_POINTS * MAX_LIMBS) + threadIdx.x] = s_g[threadIdx.x];
+  }
+}
+
+// One block per entry: sum the entry's tile partials, store g_round, derive r_round.
+template <class F, int PMAX>
+__global__ void __launch_bounds__(BLOCK) batch_finalize_kernel(BatchArgs a) {
+  constexpr int L = F::L;
+  __shared__ uint32_t s_red[(BLOCK / 64) * PMAX * L];
+  __shared__ uint32_t s_g[PMAX * L];
+  __shared__ uint32_t s_msg[3 * 34];
+  __shared__ uint32_t s_r[L];
+  const StageEntry ent = a.entries[blockIdx.x];
+  const InstDesc& in = a.insts[ent.inst];
+  const int npts = in.comp->npts;
+  Elem<L> acc[PMAX];
+#pragma unroll
+  for (int t = 0; t < PMAX; ++t) acc[t] = zero<F>();
+  for (uint32_t b = threadIdx.x; b < ent.tiles; b += BLOCK) {
+    const uint32_t* part = a.partials + (uint64_t)(ent.tile_begin + b) * (MAX_POINTS * MAX_LIMBS);
+#pragma unroll
+    for (int t = 0; t < PMAX; ++t) {
+      if (t < npts) acc[t] = add<F>(acc[t], load_elem<L>(part + t * L));
+    }
+  }
+  block_reduce<F, PMAX, BLOCK>(acc, npts, s_red, s_g);
+  finish_round<F, BLOCK>(s_g, npts, (int)ent.round, in.num_vars, in.challenge_mode, a.fc, in.st, nullptr,
+                         s_msg, s_r);
+}
+
 // Input generation and Montgomery encoding (cuda/src/field_sumcheck.cu:593-611).
 
 template <class F>

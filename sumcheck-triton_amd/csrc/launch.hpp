@@ -19,6 +19,9 @@ struct Launchers {
                          uint32_t* out, const FieldConsts* fc);
   hipError_t (*transcript)(hipStream_t, const uint32_t* g, int npts, int round,
                            const FieldConsts* fc, ProofState* st);
+  // batched stages: grid = tiles of the stage / entries of the stage
+  hipError_t (*batch_pass)(hipStream_t, int tmax, int pmax, const BatchArgs&, unsigned tiles);
+  hipError_t (*batch_finalize)(hipStream_t, int pmax, const BatchArgs&, unsigned entries);
 };
 
 const Launchers& launchers_p32();

@@ -134,4 +134,36 @@ struct FillArgs {
   const FieldConsts* fc;
 };
 
+// ---------------------------------------------------------------------------------------------
+// Batched proving: many independent instances advance round by round. One stage launch covers
+// a list of entries (instance, its current round, first tile); each block works on one tile of
+// one entry, so the composition it interprets is uniform across the block.
+// ---------------------------------------------------------------------------------------------
+struct InstDesc {
+  const uint32_t* input;  // round-0 tables, element-major Montgomery
+  uint32_t* buf_a;        // odd rounds' tables (n0/2 per table)
+  uint32_t* buf_b;        // even rounds' tables (n0/4 per table)
+  uint64_t n0;
+  int32_t num_vars;
+  int32_t challenge_mode;
+  const CompDesc* comp;
+  ProofState* st;
+};
+
+struct StageEntry {
+  uint32_t inst;
+  uint32_t round;
+  uint32_t tile_begin;  // first tile (block) of this entry in the stage
+  uint32_t tiles;
+};
+
+struct BatchArgs {
+  const InstDesc* insts;
+  const StageEntry* entries;  // sorted by tile_begin
+  uint32_t n_entries;
+  uint32_t items;             // pairs per thread in a tile: a tile is BLOCK * items pairs
+  uint32_t* partials;         // [tiles][MAX_POINTS][MAX_LIMBS]
+  const FieldConsts* fc;
+};
+
 }  // namespace sc

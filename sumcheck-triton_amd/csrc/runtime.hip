@@ -7,6 +7,7 @@
 // with the per-point D2H copies and host hashing replaced by device finalize kernels.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -754,6 +755,336 @@ int sc_calibrate(sc_context* ctx, double* rates, int32_t n) {
   SC_HIP(hipSetDevice(ctx->device));
   for (int k = 0; k < n; ++k) rates[k] = 0.0;
   SC_HIP(calibrate(ctx->stream, rates, n));
+  return SC_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Batched proving
+// ---------------------------------------------------------------------------------------------
+
+}  // extern "C"
+
+#define SC_HIP_B(expr)                                                                        \
+  do {                                                                                        \
+    hipError_t _e = (expr);                                                                   \
+    if (_e != hipSuccess) {                                                                   \
+      free_batch(b);                                                                          \
+      return fail(_e == hipErrorOutOfMemory ? SC_ERR_MEMORY : SC_ERR_HIP,                     \
+                  std::string(#expr) + ": " + hipGetErrorString(_e));                         \
+    }                                                                                         \
+  } while (0)
+
+struct sc_batch {
+  sc_context* ctx = nullptr;
+  int fidx = 0, L = 0, n = 0, tmax = 0, pmax = 0, max_rounds = 0;
+  const Launchers* ln = nullptr;
+  std::vector<int> num_vars, npts;
+  std::vector<uint64_t> n0;
+  uint8_t* pool = nullptr;
+  uint64_t pool_bytes = 0;
+  InstDesc* d_insts = nullptr;
+  CompDesc* d_comps = nullptr;
+  ProofState* d_states = nullptr;
+  StageEntry* d_entries = nullptr;
+  uint32_t* d_partials = nullptr;
+  std::vector<uint32_t> stage_offset, stage_entries, stage_tiles;
+  uint32_t items = 4;
+  int mode = -1;  // challenge mode baked into the instance descriptors
+  std::vector<InstDesc> insts;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+};
+
+namespace {
+
+void free_batch(sc_batch* b) {
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->pool) (void)hipFree(b->pool);
+  if (b->d_insts) (void)hipFree(b->d_insts);
+  if (b->d_comps) (void)hipFree(b->d_comps);
+  if (b->d_states) (void)hipFree(b->d_states);
+  if (b->d_entries) (void)hipFree(b->d_entries);
+  if (b->d_partials) (void)hipFree(b->d_partials);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  delete b;
+}
+
+void drop_batch_graph(sc_batch* b) {
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->graph_exec = nullptr;
+  b->graph = nullptr;
+}
+
+// Work lists of every stage for the current tile size, uploaded to the device.
+int build_stages(sc_batch* b) {
+  std::vector<StageEntry> all;
+  b->stage_offset.assign(b->max_rounds, 0);
+  b->stage_entries.assign(b->max_rounds, 0);
+  b->stage_tiles.assign(b->max_rounds, 0);
+  uint64_t max_tiles = 1;
+  const uint64_t tile_pairs = (uint64_t)BLOCK * b->items;
+  for (int s = 0; s < b->max_rounds; ++s) {
+    b->stage_offset[s] = (uint32_t)all.size();
+    uint64_t tiles = 0;
+    for (int i = 0; i < b->n; ++i) {
+      if (s >= b->num_vars[i]) continue;
+      const uint64_t pairs = b->n0[i] >> (s + 1);
+      const uint64_t t = (pairs + tile_pairs - 1) / tile_pairs;
+      if (tiles + t > 0x7FFFFFFFull) return fail(SC_ERR_UNSUPPORTED, "batch stage has too many tiles");
+      all.push_back(StageEntry{(uint32_t)i, (uint32_t)s, (uint32_t)tiles, (uint32_t)t});
+      tiles += t;
+    }
+    b->stage_entries[s] = (uint32_t)all.size() - b->stage_offset[s];
+    b->stage_tiles[s] = (uint32_t)tiles;
+    if (tiles > max_tiles) max_tiles = tiles;
+  }
+  if (b->d_entries) (void)hipFree(b->d_entries);
+  if (b->d_partials) (void)hipFree(b->d_partials);
+  b->d_entries = nullptr;
+  b->d_partials = nullptr;
+  SC_HIP(hipMalloc(&b->d_entries, sizeof(StageEntry) * (all.empty() ? 1 : all.size())));
+  SC_HIP(hipMalloc(&b->d_partials, max_tiles * MAX_POINTS * MAX_LIMBS * 4));
+  if (!all.empty())
+    SC_HIP(hipMemcpy(b->d_entries, all.data(), sizeof(StageEntry) * all.size(), hipMemcpyHostToDevice));
+  drop_batch_graph(b);
+  return SC_OK;
+}
+
+int set_batch_mode(sc_batch* b, int mode) {
+  if (mode != SC_CHALLENGE_FIXED && mode != SC_CHALLENGE_SHA3)
+    return fail(SC_ERR_ARG, "batched proofs take challenge_mode 0 (fixed) or 1 (sha3)");
+  if (mode == b->mode) return SC_OK;
+  for (auto& d : b->insts) d.challenge_mode = mode;
+  SC_HIP(hipMemcpy(b->d_insts, b->insts.data(), sizeof(InstDesc) * b->n, hipMemcpyHostToDevice));
+  b->mode = mode;
+  drop_batch_graph(b);
+  return SC_OK;
+}
+
+int enqueue_batch(sc_batch* b) {
+  hipStream_t s = b->ctx->stream;
+  for (int st = 0; st < b->max_rounds; ++st) {
+    if (b->stage_entries[st] == 0) continue;
+    BatchArgs a;
+    a.insts = b->d_insts;
+    a.entries = b->d_entries + b->stage_offset[st];
+    a.n_entries = b->stage_entries[st];
+    a.items = b->items;
+    a.partials = b->d_partials;
+    a.fc = b->ctx->d_fc[b->fidx];
+    SC_HIP(b->ln->batch_pass(s, b->tmax, b->pmax, a, b->stage_tiles[st]));
+    SC_HIP(b->ln->batch_finalize(s, b->pmax, a, b->stage_entries[st]));
+  }
+  return SC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sc_batch_create(sc_context* ctx, int32_t bit_width, int32_t n, const sc_composition* comps,
+                    const int32_t* num_vars, const sc_input* inputs, sc_batch** out) {
+  if (ctx == nullptr || comps == nullptr || num_vars == nullptr || inputs == nullptr || out == nullptr)
+    return fail(SC_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  const int fidx = field_index(bit_width);
+  if (fidx < 0) return fail(SC_ERR_ARG, "bit_width must be 32, 64, 128 or 256");
+  if (n < 1 || n > 65536) return fail(SC_ERR_ARG, "instance count must be in [1, 65536]");
+  std::vector<CompDesc> descs(n);
+  sc_batch* b = new (std::nothrow) sc_batch();
+  if (b == nullptr) return fail(SC_ERR_MEMORY, "host allocation failed");
+  b->ctx = ctx;
+  b->fidx = fidx;
+  b->ln = &launchers_for(fidx);
+  b->L = b->ln->limbs;
+  b->n = n;
+  b->num_vars.resize(n);
+  b->npts.resize(n);
+  b->n0.resize(n);
+  int maxT = 0, maxP = 0;
+  std::vector<uint64_t> offs(n);
+  uint64_t total = 0;
+  auto align = [](uint64_t x) { return (x + 255) & ~(uint64_t)255; };
+  for (int i = 0; i < n; ++i) {
+    int degree = 0;
+    int rc = build_comp_any(fidx, &comps[i], ctx->host_fc[fidx], descs[i], degree);
+    if (rc != SC_OK) {
+      free_batch(b);
+      return rc;
+    }
+    if (num_vars[i] < 1 || num_vars[i] > MAX_ROUNDS) {
+      free_batch(b);
+      return fail(SC_ERR_UNSUPPORTED, "num_vars must be in [1, 40]");
+    }
+    if (inputs[i].kind != SC_INPUT_AFFINE && inputs[i].kind != SC_INPUT_UNIFORM) {
+      free_batch(b);
+      return fail(SC_ERR_ARG, "batched instances take generated inputs (affine or uniform)");
+    }
+    b->num_vars[i] = num_vars[i];
+    b->npts[i] = descs[i].npts;
+    b->n0[i] = (uint64_t)1 << num_vars[i];
+    maxT = std::max(maxT, descs[i].num_tables);
+    maxP = std::max(maxP, descs[i].npts);
+    b->max_rounds = std::max(b->max_rounds, num_vars[i]);
+    offs[i] = total;
+    const uint64_t words = (uint64_t)descs[i].num_tables * b->n0[i] * b->L;
+    total += align(words * 4) + align(words / 2 * 4) + align(words / 4 * 4 + 4);
+  }
+  b->tmax = pick_tmax(maxT);
+  b->pmax = maxP <= 4 ? 4 : 8;
+  SC_HIP_B(hipSetDevice(ctx->device));
+  SC_HIP_B(hipMalloc(&b->pool, total));
+  b->pool_bytes = total;
+  SC_HIP_B(hipMalloc(&b->d_insts, sizeof(InstDesc) * n));
+  SC_HIP_B(hipMalloc(&b->d_comps, sizeof(CompDesc) * n));
+  SC_HIP_B(hipMalloc(&b->d_states, sizeof(ProofState) * n));
+  SC_HIP_B(hipMemcpy(b->d_comps, descs.data(), sizeof(CompDesc) * n, hipMemcpyHostToDevice));
+  SC_HIP_B(hipMemset(b->d_states, 0, sizeof(ProofState) * n));
+  SC_HIP_B(hipEventCreate(&b->ev0));
+  SC_HIP_B(hipEventCreate(&b->ev1));
+  b->insts.resize(n);
+  for (int i = 0; i < n; ++i) {
+    const uint64_t words = (uint64_t)descs[i].num_tables * b->n0[i] * b->L;
+    uint8_t* base = b->pool + offs[i];
+    InstDesc& d = b->insts[i];
+    d.input = (const uint32_t*)base;
+    d.buf_a = (uint32_t*)(base + align(words * 4));
+    d.buf_b = (uint32_t*)(base + align(words * 4) + align(words / 2 * 4));
+    d.n0 = b->n0[i];
+    d.num_vars = num_vars[i];
+    d.challenge_mode = SC_CHALLENGE_SHA3;
+    d.comp = b->d_comps + i;
+    d.st = b->d_states + i;
+    FillArgs f;
+    f.tables = (uint32_t*)base;
+    f.n = b->n0[i];
+    f.num_tables = descs[i].num_tables;
+    f.kind = inputs[i].kind == SC_INPUT_AFFINE ? FILL_AFFINE : FILL_UNIFORM;
+    f.seed = inputs[i].seed;
+    f.selector_mask = inputs[i].selector_mask;
+    f.instance = inputs[i].instance;
+    f.pad0 = 0;
+    f.fc = ctx->d_fc[fidx];
+    SC_HIP_B(b->ln->fill(ctx->stream, f));
+  }
+  SC_HIP_B(hipMemcpy(b->d_insts, b->insts.data(), sizeof(InstDesc) * n, hipMemcpyHostToDevice));
+  b->mode = SC_CHALLENGE_SHA3;
+  SC_HIP_B(hipStreamSynchronize(ctx->stream));
+  int rc = build_stages(b);
+  if (rc != SC_OK) {
+    free_batch(b);
+    return rc;
+  }
+  *out = b;
+  return SC_OK;
+}
+
+int sc_batch_set_tile(sc_batch* b, int32_t items_per_thread) {
+  if (b == nullptr) return fail(SC_ERR_ARG, "batch must not be NULL");
+  if (items_per_thread < 1 || items_per_thread > 64) return fail(SC_ERR_ARG, "items_per_thread must be in [1, 64]");
+  if ((uint32_t)items_per_thread == b->items) return SC_OK;
+  b->items = (uint32_t)items_per_thread;
+  SC_HIP(hipStreamSynchronize(b->ctx->stream));
+  return build_stages(b);
+}
+
+int sc_batch_run(sc_batch* b, int32_t mode) {
+  if (b == nullptr) return fail(SC_ERR_ARG, "batch must not be NULL");
+  SC_HIP(hipSetDevice(b->ctx->device));
+  SC_HIP(hipStreamSynchronize(b->ctx->stream));
+  int rc = set_batch_mode(b, mode);
+  if (rc != SC_OK) return rc;
+  hipStream_t s = b->ctx->stream;
+  SC_HIP(hipEventRecord(b->ev0, s));
+  if (b->graph_exec) {
+    SC_HIP(hipGraphLaunch(b->graph_exec, s));
+  } else {
+    rc = enqueue_batch(b);
+    if (rc != SC_OK) return rc;
+  }
+  SC_HIP(hipEventRecord(b->ev1, s));
+  return SC_OK;
+}
+
+int sc_batch_capture(sc_batch* b, int32_t mode) {
+  if (b == nullptr) return fail(SC_ERR_ARG, "batch must not be NULL");
+  SC_HIP(hipSetDevice(b->ctx->device));
+  SC_HIP(hipStreamSynchronize(b->ctx->stream));
+  int rc = set_batch_mode(b, mode);
+  if (rc != SC_OK) return rc;
+  drop_batch_graph(b);
+  hipStream_t s = b->ctx->stream;
+  SC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  rc = enqueue_batch(b);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc != SC_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(SC_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  b->graph = g;
+  SC_HIP(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+  return SC_OK;
+}
+
+int sc_batch_wait(sc_batch* b, float* ms) {
+  if (b == nullptr) return fail(SC_ERR_ARG, "batch must not be NULL");
+  SC_HIP(hipSetDevice(b->ctx->device));
+  SC_HIP(hipStreamSynchronize(b->ctx->stream));
+  if (ms != nullptr) {
+    *ms = 0.0f;
+    SC_HIP(hipEventElapsedTime(ms, b->ev0, b->ev1));
+  }
+  return SC_OK;
+}
+
+int sc_batch_result(sc_batch* b, int32_t i, uint32_t* round_polys, uint32_t* challenges, uint64_t* checksum) {
+  if (b == nullptr) return fail(SC_ERR_ARG, "batch must not be NULL");
+  if (i < 0 || i >= b->n) return fail(SC_ERR_ARG, "instance index out of range");
+  SC_HIP(hipSetDevice(b->ctx->device));
+  SC_HIP(hipStreamSynchronize(b->ctx->stream));
+  std::vector<ProofState> host(1);
+  SC_HIP(hipMemcpy(host.data(), b->d_states + i, sizeof(ProofState), hipMemcpyDeviceToHost));
+  const ProofState& st = host[0];
+  const int npts = b->npts[i], L = b->L, nv = b->num_vars[i];
+  uint64_t sum = 0;
+  for (int j = 0; j < nv; ++j) {
+    for (int t = 0; t < npts; ++t) {
+      for (int k = 0; k < L; ++k) {
+        const uint32_t w = st.polys[j][t][k];
+        sum += (uint64_t)(k + 1) * w;
+        if (round_polys) round_polys[((size_t)j * npts + t) * L + k] = w;
+      }
+    }
+    if (challenges && j + 1 < nv)
+      for (int k = 0; k < L; ++k) challenges[(size_t)j * L + k] = st.r[j][k];
+  }
+  if (checksum) *checksum = sum;
+  return SC_OK;
+}
+
+int sc_batch_info(sc_batch* b, uint64_t* device_bytes, int32_t* stages, uint64_t* tiles) {
+  if (b == nullptr) return fail(SC_ERR_ARG, "batch must not be NULL");
+  if (device_bytes) *device_bytes = b->pool_bytes;
+  if (stages) *stages = b->max_rounds;
+  if (tiles) {
+    uint64_t t = 0;
+    for (uint32_t x : b->stage_tiles) t += x;
+    *tiles = t;
+  }
+  return SC_OK;
+}
+
+int sc_batch_destroy(sc_batch* b) {
+  if (b == nullptr) return SC_OK;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  free_batch(b);
   return SC_OK;
 }
 

@@ -85,6 +85,16 @@ SIGNATURES = [
       ctypes.POINTER(ScInput), ctypes.c_int32, u32p, u32p, u32p, u64p]),
     ("sc_field_op", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, u32p, u32p, u32p]),
+    ("sc_batch_create", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ScComposition), i32p,
+      ctypes.POINTER(ScInput), ctypes.POINTER(ctypes.c_void_p)]),
+    ("sc_batch_set_tile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("sc_batch_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("sc_batch_capture", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("sc_batch_wait", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    ("sc_batch_result", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, u32p, u32p, u64p]),
+    ("sc_batch_info", ctypes.c_int, [ctypes.c_void_p, u64p, i32p, u64p]),
+    ("sc_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("sc_calibrate", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int32]),
     ("sc_transcript_challenge", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, u32p, ctypes.c_int32, u32p]),

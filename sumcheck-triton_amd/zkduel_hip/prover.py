@@ -291,3 +291,85 @@ class Proof:
         g = [[f.montgomery_decode(v) for v in f.limbs_to_ints(polys[j])] for j in range(self.num_vars)]
         r = [f.montgomery_decode(v) for v in f.limbs_to_ints(chal)] if self.num_vars > 1 else []
         return g, r, cs
+
+
+class Batch:
+    """Many independent instances proved together, round by round (sc_batch).
+
+    ``instances`` is a list of ``(composition, num_vars, input)`` where input is a dict for
+    the generator: ``{"kind": "uniform"|"affine", "seed": int, "instance": int,
+    "selector_mask": int}``.
+    """
+
+    def __init__(self, ctx: Context, bit_width: int, instances: Sequence[tuple]):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.field = get_field(bit_width)
+        n = len(instances)
+        if n == 0:
+            raise ValueError("a batch needs at least one instance")
+        self.comps = [c for c, _nv, _inp in instances]
+        self.num_vars = [int(nv) for _c, nv, _inp in instances]
+        bound = [c.bind(self.field) for c in self.comps]
+        self._keep = [k for _s, k in bound]
+        carr = (_lib.ScComposition * n)(*[s for s, _k in bound])
+        nvs = np.asarray(self.num_vars, dtype=np.int32)
+        inps = (_lib.ScInput * n)()
+        for i, (_c, _nv, inp) in enumerate(instances):
+            kind = INPUT_KINDS[inp.get("kind", "uniform")]
+            inps[i] = _lib.ScInput(kind, int(inp.get("instance", i)) & 0xFFFFFFFF, None,
+                                   int(inp.get("seed", 1)) & ((1 << 64) - 1),
+                                   int(inp.get("selector_mask", 0)) & ((1 << 64) - 1))
+        handle = ctypes.c_void_p()
+        _lib.check(self.lib.sc_batch_create(ctx.handle, self.field.bit_width, n, carr,
+                                            nvs.ctypes.data_as(_lib.i32p), inps, ctypes.byref(handle)))
+        self.handle = handle
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.sc_batch_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __enter__(self) -> "Batch":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self) -> int:
+        return len(self.comps)
+
+    def set_tile(self, items_per_thread: int) -> None:
+        _lib.check(self.lib.sc_batch_set_tile(self.handle, int(items_per_thread)))
+
+    def run(self, challenge_mode: str = "sha3") -> None:
+        _lib.check(self.lib.sc_batch_run(self.handle, _lib.CHALLENGE_MODES[challenge_mode]))
+
+    def capture(self, challenge_mode: str = "sha3") -> None:
+        _lib.check(self.lib.sc_batch_capture(self.handle, _lib.CHALLENGE_MODES[challenge_mode]))
+
+    def wait(self) -> float:
+        ms = ctypes.c_float()
+        _lib.check(self.lib.sc_batch_wait(self.handle, ctypes.byref(ms)))
+        return float(ms.value)
+
+    def result(self, i: int) -> tuple[np.ndarray, np.ndarray, int]:
+        L = self.field.limbs
+        nv = self.num_vars[i]
+        polys = np.zeros((nv, self.comps[i].degree + 1, L), dtype=np.uint32)
+        chal = np.zeros((max(nv - 1, 1), L), dtype=np.uint32)
+        cs = ctypes.c_uint64()
+        _lib.check(self.lib.sc_batch_result(self.handle, int(i), _lib.as_u32p(polys), _lib.as_u32p(chal),
+                                            ctypes.byref(cs)))
+        return polys, chal[: nv - 1], int(cs.value)
+
+    def info(self) -> dict:
+        nb, st, tl = ctypes.c_uint64(), ctypes.c_int32(), ctypes.c_uint64()
+        _lib.check(self.lib.sc_batch_info(self.handle, ctypes.byref(nb), ctypes.byref(st), ctypes.byref(tl)))
+        return {"device_bytes": nb.value, "stages": st.value, "tiles": tl.value}
