@@ -114,6 +114,77 @@ def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: floa
     }
 
 
+def measure_plain(ctx, bw: int, comp, nv: int, reps: int, selector_mask: int = 0) -> dict:
+    """Median whole-proof time of one prepared instance (SHA3, uniform tables)."""
+    from zkduel_hip import Proof
+
+    with Proof(ctx, bw, comp, nv, input="uniform", seed=SEED, selector_mask=selector_mask) as pr:
+        pr.run("sha3")
+        pr.wait()
+        lat = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            pr.run("sha3")
+            pr.wait()
+            pr.result()
+            lat.append((time.perf_counter() - t0) * 1000.0)
+    lat.sort()
+    return {"bit_width": bw, "num_vars": nv, "ms_per_proof": statistics.median(lat),
+            "proofs_per_s": 1000.0 / statistics.median(lat), "p99_ms": lat[-1]}
+
+
+def measure_batched(ctx, bw: int, n: int = 256, reps: int = 3) -> dict:
+    """Config 4: n uneven instances (num_vars 10..22, random product sums over 2..8 tables),
+    proved one at a time (plain) and as one batch; both SHA3. The tile size of the batch is
+    picked by timing candidates (autotune) before the timed runs; results must agree."""
+    from zkduel_hip import Proof
+    from zkduel_hip.prover import Batch
+    from zkduel_hip.synthetic import batch_instances
+
+    insts = batch_instances(n, seed=4)
+    with Batch(ctx, bw, insts) as b:
+        tune = {}
+        for items in (1, 2, 4, 8):
+            b.set_tile(items)
+            b.run("sha3")
+            b.wait()
+            b.run("sha3")
+            tune[items] = b.wait()
+        best = min(tune, key=tune.get)
+        b.set_tile(best)
+        b.capture("sha3")
+        b.run("sha3")
+        b.wait()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            b.run("sha3")
+            b.wait()
+        batched_s = (time.perf_counter() - t0) / reps
+        batch_cs = [b.result(i)[2] for i in range(n)]
+        info = b.info()
+    proofs = [Proof(ctx, bw, c, nv, input="uniform", seed=inp["seed"], instance=inp["instance"]) for c, nv, inp in insts]
+    try:
+        for pr in proofs:
+            pr.run("sha3")
+            pr.wait()
+        ctx.sync()
+        t0 = time.perf_counter()
+        plain_cs = []
+        for pr in proofs:
+            pr.run("sha3")
+            pr.wait()
+            plain_cs.append(pr.result()[2])
+        plain_s = time.perf_counter() - t0
+    finally:
+        for pr in proofs:
+            pr.close()
+    return {"instances": n, "bit_width": bw, "tile_items_autotuned": best,
+            "tile_candidates_ms": {str(k): round(v, 3) for k, v in tune.items()},
+            "batched_proofs_per_s": n / batched_s, "plain_proofs_per_s": n / plain_s,
+            "speedup": plain_s / batched_s, "batched_ms": batched_s * 1000.0, "plain_ms": plain_s * 1000.0,
+            "results_identical": batch_cs == plain_cs, "device_bytes": info["device_bytes"]}
+
+
 def cpu_baseline(budget_s: float = 20.0) -> dict:
     """Oracle (single-core Python restatement of reference.py) on the same workload at the
     largest num_vars that fits the budget; proofs/s scaled to num_vars 26 by 2^(nv - 26)
@@ -160,6 +231,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--widths", type=int, nargs="+", default=[64, 32, 128, 256])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -181,6 +253,20 @@ def main() -> int:
         log(f"bench: width {bw}: {per[bw]['proofs_per_s']:.1f} proofs/s, "
             f"pass kernel {per[bw]['pass_kernel_gbps']:.0f} GB/s")
     head = per[64]
+    from zkduel_hip import Composition, get_spec
+    from zkduel_hip.synthetic import constraint_composition
+
+    log("bench: config 1 (Goldilocks a*b*c, num_vars 16)")
+    cfg1 = measure_plain(ctx, 64, Composition.from_spec(get_spec("poly_abc")), 16, max(5, args.steps))
+    log("bench: config 3 (BLS12-381, 10 + 2 selector tables, 12 monomials of degree 3-6, num_vars 22)")
+    comp3, mask3 = constraint_composition(256)
+    cfg3 = measure_plain(ctx, 256, comp3, 22, 2, selector_mask=mask3)
+    batched = {}
+    if not args.no_batch:
+        for bw in (64, 256):
+            log(f"bench: config 4 batched vs plain, {bw}-bit")
+            batched[str(bw)] = measure_batched(ctx, bw)
+            log(f"bench:   {batched[str(bw)]['batched_proofs_per_s']:.0f} vs {batched[str(bw)]['plain_proofs_per_s']:.0f} proofs/s")
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")
@@ -211,6 +297,10 @@ def main() -> int:
                 str(bw): {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
                 for bw, r in per.items()
             },
+            "config1_goldilocks_abc_nv16": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in cfg1.items()},
+            "config3_bls_constraint_nv22": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in cfg3.items()},
+            "config4_batched_vs_plain": {bw: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+                                         for bw, r in batched.items()},
             "calibration": {
                 "mad_u64_u32_lane_ops_per_s": calib["mad_u64_u32"],
                 "hbm_read_gbps": calib["hbm_read"] / 1e9,

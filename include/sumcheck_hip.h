@@ -162,6 +162,26 @@ int sc_batch_result(sc_batch* batch, int32_t instance, uint32_t* round_polys, ui
 int sc_batch_info(sc_batch* batch, uint64_t* device_bytes, int32_t* stages, uint64_t* tiles);
 int sc_batch_destroy(sc_batch* batch);
 
+/* Serving (no counterpart in the reference): continuous batching of proof requests. Requests
+ * are queued by sc_server_submit and admitted at step boundaries, first in first out, while a
+ * slot (max_active) and HBM (hbm_budget_bytes, the arena holding their tables: 1.75x input)
+ * are free. Each sc_server_step enqueues one round of every running request, each at its own
+ * round, using the batched stage kernels grouped by width; tables and transcript state stay
+ * on the device between steps. A request's result is the same bits as sc_prove on it.
+ * Results are read once (sc_server_result frees the request). Poll states: 0 queued,
+ * 1 running, 2 done. Calls on one server must be serialized. */
+typedef struct sc_server sc_server;
+int sc_server_create(sc_context* ctx, uint64_t hbm_budget_bytes, int32_t max_active, sc_server** out);
+int sc_server_submit(sc_server* server, int32_t bit_width, const sc_composition* comp, int32_t num_vars,
+                     const sc_input* input, int32_t challenge_mode, int64_t* handle);
+int sc_server_step(sc_server* server, int32_t* n_running, int32_t* n_queued);
+int sc_server_poll(sc_server* server, int64_t handle, int32_t* state);
+int sc_server_drain(sc_server* server);
+int sc_server_result(sc_server* server, int64_t handle, uint32_t* round_polys, uint32_t* challenges,
+                     uint64_t* checksum);
+int sc_server_info(sc_server* server, uint64_t* arena_used, uint64_t* arena_size, uint64_t* steps);
+int sc_server_destroy(sc_server* server);
+
 /* Measured sustained lane rates (operations per second) of the VALU operations the field
  * arithmetic is built from: rates[0] v_mad_u64_u32 (32x32+64), rates[1] v_mul_hi_u32,
  * rates[2] 64-bit add, rates[3] v_fma_f64; and streaming HBM bandwidth in bytes/s:

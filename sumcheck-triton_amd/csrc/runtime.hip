@@ -9,6 +9,8 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <deque>
+#include <map>
 #include <cstring>
 #include <new>
 #include <string>
@@ -1085,6 +1087,449 @@ int sc_batch_destroy(sc_batch* b) {
   (void)hipSetDevice(b->ctx->device);
   (void)hipStreamSynchronize(b->ctx->stream);
   free_batch(b);
+  return SC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// Serving: continuous batching of independent proof requests
+// ---------------------------------------------------------------------------------------------
+
+namespace {
+
+// First-fit arena over one device allocation; offsets are 256-byte aligned.
+struct Arena {
+  uint8_t* base = nullptr;
+  uint64_t size = 0;
+  std::map<uint64_t, uint64_t> free_blocks;  // offset -> length
+  uint64_t used = 0;
+
+  void init(uint8_t* b, uint64_t n) {
+    base = b;
+    size = n;
+    free_blocks.clear();
+    free_blocks[0] = n;
+    used = 0;
+  }
+  bool alloc(uint64_t n, uint64_t& off) {
+    n = (n + 255) & ~(uint64_t)255;
+    for (auto it = free_blocks.begin(); it != free_blocks.end(); ++it) {
+      if (it->second >= n) {
+        off = it->first;
+        const uint64_t rest = it->second - n;
+        free_blocks.erase(it);
+        if (rest) free_blocks[off + n] = rest;
+        used += n;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(uint64_t off, uint64_t n) {
+    n = (n + 255) & ~(uint64_t)255;
+    used -= n;
+    auto it = free_blocks.emplace(off, n).first;
+    auto next = std::next(it);
+    if (next != free_blocks.end() && it->first + it->second == next->first) {
+      it->second += next->second;
+      free_blocks.erase(next);
+    }
+    if (it != free_blocks.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == it->first) {
+        prev->second += it->second;
+        free_blocks.erase(it);
+      }
+    }
+  }
+};
+
+enum ReqState { REQ_QUEUED = 0, REQ_RUNNING = 1, REQ_DONE = 2, REQ_FINISHING = 3 };
+
+struct Request {
+  int64_t id = 0;
+  int fidx = 0, L = 0, npts = 0, num_vars = 0, mode = SC_CHALLENGE_SHA3;
+  CompDesc desc;
+  sc_input input;
+  int state = REQ_QUEUED;
+  int slot = -1;
+  int round = 0;  // next round to run
+  uint64_t arena_off = 0, arena_bytes = 0;
+  ProofState* result = nullptr;  // pinned host copy of the final state
+};
+
+struct StepRecord {
+  hipEvent_t ev = nullptr;
+  std::vector<int64_t> finishing;
+  int staging = -1;
+};
+
+}  // namespace
+
+struct sc_server {
+  sc_context* ctx = nullptr;
+  int max_active = 0;
+  uint32_t items = 4;
+  uint8_t* arena_mem = nullptr;
+  Arena arena;
+  InstDesc* d_insts = nullptr;       // [max_active]
+  CompDesc* d_comps = nullptr;       // [max_active]
+  ProofState* d_states = nullptr;    // [max_active]
+  StageEntry* d_entries = nullptr;   // [max_active]
+  uint32_t* d_partials = nullptr;    // [max_tiles][MAX_POINTS][MAX_LIMBS]
+  uint64_t max_tiles = 0;
+  std::vector<int> free_slots;
+  std::deque<int64_t> queue;
+  std::map<int64_t, Request> reqs;
+  std::vector<int64_t> slot_req;
+  int64_t next_id = 1;
+  // pinned staging ring for descriptors and entry lists
+  struct Staging {
+    InstDesc* insts = nullptr;
+    CompDesc* comps = nullptr;
+    StageEntry* entries = nullptr;
+    hipEvent_t ev = nullptr;
+    bool busy = false;
+  };
+  std::vector<Staging> staging;
+  std::deque<StepRecord> inflight;
+  uint64_t steps = 0;
+  size_t max_inflight = 3;  // steps enqueued ahead of the device before the host waits
+};
+
+namespace {
+
+void free_server(sc_server* sv) {
+  if (sv->ctx) (void)hipStreamSynchronize(sv->ctx->stream);
+  for (auto& kv : sv->reqs)
+    if (kv.second.result) (void)hipHostFree(kv.second.result);
+  for (auto& st : sv->staging) {
+    if (st.insts) (void)hipHostFree(st.insts);
+    if (st.comps) (void)hipHostFree(st.comps);
+    if (st.entries) (void)hipHostFree(st.entries);
+    if (st.ev) (void)hipEventDestroy(st.ev);
+  }
+  for (auto& r : sv->inflight)
+    if (r.ev) (void)hipEventDestroy(r.ev);
+  if (sv->arena_mem) (void)hipFree(sv->arena_mem);
+  if (sv->d_insts) (void)hipFree(sv->d_insts);
+  if (sv->d_comps) (void)hipFree(sv->d_comps);
+  if (sv->d_states) (void)hipFree(sv->d_states);
+  if (sv->d_entries) (void)hipFree(sv->d_entries);
+  if (sv->d_partials) (void)hipFree(sv->d_partials);
+  delete sv;
+}
+
+uint64_t request_bytes(const Request& r) {
+  const uint64_t words = (uint64_t)r.desc.num_tables * ((uint64_t)1 << r.num_vars) * r.L;
+  auto align = [](uint64_t x) { return (x + 255) & ~(uint64_t)255; };
+  return align(words * 4) + align(words / 2 * 4) + align(words / 4 * 4 + 4);
+}
+
+// A free staging buffer, waiting for the oldest if all are in flight.
+int take_staging(sc_server* sv) {
+  for (size_t i = 0; i < sv->staging.size(); ++i) {
+    auto& st = sv->staging[i];
+    if (!st.busy) return (int)i;
+    if (hipEventQuery(st.ev) == hipSuccess) {
+      st.busy = false;
+      return (int)i;
+    }
+  }
+  // all busy: wait on each in turn
+  for (size_t i = 0; i < sv->staging.size(); ++i) {
+    if (hipEventSynchronize(sv->staging[i].ev) == hipSuccess) {
+      sv->staging[i].busy = false;
+      return (int)i;
+    }
+  }
+  return -1;
+}
+
+// Retire steps whose work finished: finishing requests become done, their memory and slot
+// return to the pools. wait = block for every in-flight step.
+int retire(sc_server* sv, bool wait) {
+  while (!sv->inflight.empty()) {
+    StepRecord& rec = sv->inflight.front();
+    if (wait) {
+      SC_HIP(hipEventSynchronize(rec.ev));
+    } else if (hipEventQuery(rec.ev) != hipSuccess) {
+      break;
+    }
+    for (int64_t id : rec.finishing) {
+      Request& r = sv->reqs[id];
+      r.state = REQ_DONE;
+      sv->arena.release(r.arena_off, r.arena_bytes);
+      sv->free_slots.push_back(r.slot);
+      sv->slot_req[r.slot] = 0;
+      r.slot = -1;
+    }
+    (void)hipEventDestroy(rec.ev);
+    sv->inflight.pop_front();
+  }
+  return SC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sc_server_create(sc_context* ctx, uint64_t hbm_budget_bytes, int32_t max_active, sc_server** out) {
+  if (ctx == nullptr || out == nullptr) return fail(SC_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  if (max_active < 1 || max_active > 4096) return fail(SC_ERR_ARG, "max_active must be in [1, 4096]");
+  if (hbm_budget_bytes < (1u << 20)) return fail(SC_ERR_ARG, "hbm_budget_bytes too small");
+  SC_HIP(hipSetDevice(ctx->device));
+  sc_server* sv = new (std::nothrow) sc_server();
+  if (sv == nullptr) return fail(SC_ERR_MEMORY, "host allocation failed");
+  sv->ctx = ctx;
+  sv->max_active = max_active;
+  sv->max_tiles = (uint64_t)1 << 20;
+#define SC_HIP_S(expr)                                                                        \
+  do {                                                                                        \
+    hipError_t _e = (expr);                                                                   \
+    if (_e != hipSuccess) {                                                                   \
+      free_server(sv);                                                                        \
+      return fail(_e == hipErrorOutOfMemory ? SC_ERR_MEMORY : SC_ERR_HIP,                     \
+                  std::string(#expr) + ": " + hipGetErrorString(_e));                         \
+    }                                                                                         \
+  } while (0)
+  SC_HIP_S(hipMalloc(&sv->arena_mem, hbm_budget_bytes));
+  sv->arena.init(sv->arena_mem, hbm_budget_bytes);
+  SC_HIP_S(hipMalloc(&sv->d_insts, sizeof(InstDesc) * max_active));
+  SC_HIP_S(hipMalloc(&sv->d_comps, sizeof(CompDesc) * max_active));
+  SC_HIP_S(hipMalloc(&sv->d_states, sizeof(ProofState) * max_active));
+  SC_HIP_S(hipMalloc(&sv->d_entries, sizeof(StageEntry) * max_active));
+  SC_HIP_S(hipMalloc(&sv->d_partials, sv->max_tiles * MAX_POINTS * MAX_LIMBS * 4));
+  sv->staging.resize(4);
+  for (auto& st : sv->staging) {
+    SC_HIP_S(hipHostMalloc(&st.insts, sizeof(InstDesc) * max_active, hipHostMallocDefault));
+    SC_HIP_S(hipHostMalloc(&st.comps, sizeof(CompDesc) * max_active, hipHostMallocDefault));
+    SC_HIP_S(hipHostMalloc(&st.entries, sizeof(StageEntry) * max_active, hipHostMallocDefault));
+    SC_HIP_S(hipEventCreateWithFlags(&st.ev, hipEventDisableTiming));
+  }
+#undef SC_HIP_S
+  sv->slot_req.assign(max_active, 0);
+  for (int i = max_active - 1; i >= 0; --i) sv->free_slots.push_back(i);
+  *out = sv;
+  return SC_OK;
+}
+
+int sc_server_submit(sc_server* sv, int32_t bit_width, const sc_composition* comp, int32_t num_vars,
+                     const sc_input* input, int32_t challenge_mode, int64_t* handle) {
+  if (sv == nullptr || input == nullptr || handle == nullptr) return fail(SC_ERR_ARG, "NULL argument");
+  const int fidx = field_index(bit_width);
+  if (fidx < 0) return fail(SC_ERR_ARG, "bit_width must be 32, 64, 128 or 256");
+  if (num_vars < 1 || num_vars > MAX_ROUNDS) return fail(SC_ERR_UNSUPPORTED, "num_vars must be in [1, 40]");
+  if (input->kind != SC_INPUT_AFFINE && input->kind != SC_INPUT_UNIFORM)
+    return fail(SC_ERR_ARG, "served requests take generated inputs (affine or uniform)");
+  if (challenge_mode != SC_CHALLENGE_FIXED && challenge_mode != SC_CHALLENGE_SHA3)
+    return fail(SC_ERR_ARG, "challenge_mode must be 0 (fixed) or 1 (sha3)");
+  Request r;
+  int degree = 0;
+  int rc = build_comp_any(fidx, comp, sv->ctx->host_fc[fidx], r.desc, degree);
+  if (rc != SC_OK) return rc;
+  r.id = sv->next_id++;
+  r.fidx = fidx;
+  r.L = launchers_for(fidx).limbs;
+  r.npts = r.desc.npts;
+  r.num_vars = num_vars;
+  r.mode = challenge_mode;
+  r.input = *input;
+  if (request_bytes(r) > sv->arena.size) return fail(SC_ERR_MEMORY, "request exceeds the HBM budget");
+  SC_HIP(hipHostMalloc(&r.result, sizeof(ProofState), hipHostMallocDefault));
+  sv->reqs[r.id] = r;
+  sv->queue.push_back(r.id);
+  *handle = r.id;
+  return SC_OK;
+}
+
+// One scheduling step: retire finished steps, admit queued requests that fit (FIFO), and
+// enqueue one round of every running request (batched stage launches per field width).
+int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
+  if (sv == nullptr) return fail(SC_ERR_ARG, "server must not be NULL");
+  sc_context* ctx = sv->ctx;
+  SC_HIP(hipSetDevice(ctx->device));
+  int rc = retire(sv, false);
+  if (rc != SC_OK) return rc;
+  while (sv->inflight.size() >= sv->max_inflight) {
+    SC_HIP(hipEventSynchronize(sv->inflight.front().ev));
+    rc = retire(sv, false);
+    if (rc != SC_OK) return rc;
+  }
+  hipStream_t s = ctx->stream;
+  // admit
+  while (!sv->queue.empty() && !sv->free_slots.empty()) {
+    Request& r = sv->reqs[sv->queue.front()];
+    const uint64_t bytes = request_bytes(r);
+    uint64_t off = 0;
+    if (!sv->arena.alloc(bytes, off)) break;
+    sv->queue.pop_front();
+    r.slot = sv->free_slots.back();
+    sv->free_slots.pop_back();
+    sv->slot_req[r.slot] = r.id;
+    r.arena_off = off;
+    r.arena_bytes = bytes;
+    r.state = REQ_RUNNING;
+    r.round = 0;
+    const Launchers& ln = launchers_for(r.fidx);
+    const uint64_t n0 = (uint64_t)1 << r.num_vars;
+    const uint64_t words = (uint64_t)r.desc.num_tables * n0 * r.L;
+    auto align = [](uint64_t x) { return (x + 255) & ~(uint64_t)255; };
+    uint8_t* base = sv->arena.base + off;
+    const int si = take_staging(sv);
+    if (si < 0) return fail(SC_ERR_HIP, "no staging buffer");
+    auto& st = sv->staging[si];
+    InstDesc& d = st.insts[0];
+    d.input = (const uint32_t*)base;
+    d.buf_a = (uint32_t*)(base + align(words * 4));
+    d.buf_b = (uint32_t*)(base + align(words * 4) + align(words / 2 * 4));
+    d.n0 = n0;
+    d.num_vars = r.num_vars;
+    d.challenge_mode = r.mode;
+    d.comp = sv->d_comps + r.slot;
+    d.st = sv->d_states + r.slot;
+    st.comps[0] = r.desc;
+    SC_HIP(hipMemcpyAsync(sv->d_insts + r.slot, &st.insts[0], sizeof(InstDesc), hipMemcpyHostToDevice, s));
+    SC_HIP(hipMemcpyAsync(sv->d_comps + r.slot, &st.comps[0], sizeof(CompDesc), hipMemcpyHostToDevice, s));
+    SC_HIP(hipMemsetAsync(sv->d_states + r.slot, 0, sizeof(ProofState), s));
+    SC_HIP(hipEventRecord(st.ev, s));
+    st.busy = true;
+    FillArgs f;
+    f.tables = (uint32_t*)base;
+    f.n = n0;
+    f.num_tables = r.desc.num_tables;
+    f.kind = r.input.kind == SC_INPUT_AFFINE ? FILL_AFFINE : FILL_UNIFORM;
+    f.seed = r.input.seed;
+    f.selector_mask = r.input.selector_mask;
+    f.instance = r.input.instance;
+    f.pad0 = 0;
+    f.fc = ctx->d_fc[r.fidx];
+    SC_HIP(ln.fill(s, f));
+  }
+  // one round of every running request, grouped by width
+  StepRecord rec;
+  int staged = 0;
+  bool any = false;
+  for (int fidx = 0; fidx < 4; ++fidx) {
+    std::vector<StageEntry> ents;
+    int maxT = 0, maxP = 0;
+    uint64_t tiles = 0;
+    const uint64_t tile_pairs = (uint64_t)BLOCK * sv->items;
+    for (int slot = 0; slot < sv->max_active; ++slot) {
+      const int64_t id = sv->slot_req[slot];
+      if (id == 0) continue;
+      Request& r = sv->reqs[id];
+      if (r.state != REQ_RUNNING || r.fidx != fidx) continue;
+      const uint64_t pairs = ((uint64_t)1 << r.num_vars) >> (r.round + 1);
+      const uint64_t t = (pairs + tile_pairs - 1) / tile_pairs;
+      if (tiles + t > sv->max_tiles) break;  // the rest run next step
+      ents.push_back(StageEntry{(uint32_t)slot, (uint32_t)r.round, (uint32_t)tiles, (uint32_t)t});
+      tiles += t;
+      maxT = std::max(maxT, r.desc.num_tables);
+      maxP = std::max(maxP, r.npts);
+    }
+    if (ents.empty()) continue;
+    any = true;
+    const int si = take_staging(sv);
+    if (si < 0) return fail(SC_ERR_HIP, "no staging buffer");
+    auto& st = sv->staging[si];
+    std::copy(ents.begin(), ents.end(), st.entries);
+    SC_HIP(hipMemcpyAsync(sv->d_entries, st.entries, sizeof(StageEntry) * ents.size(), hipMemcpyHostToDevice, s));
+    SC_HIP(hipEventRecord(st.ev, s));
+    st.busy = true;
+    ++staged;
+    BatchArgs a;
+    a.insts = sv->d_insts;
+    a.entries = sv->d_entries;
+    a.n_entries = (uint32_t)ents.size();
+    a.items = sv->items;
+    a.partials = sv->d_partials;
+    a.fc = ctx->d_fc[fidx];
+    const Launchers& ln = launchers_for(fidx);
+    const int tmax = pick_tmax(maxT), pmax = maxP <= 4 ? 4 : 8;
+    SC_HIP(ln.batch_pass(s, tmax, pmax, a, (unsigned)tiles));
+    SC_HIP(ln.batch_finalize(s, pmax, a, (unsigned)ents.size()));
+    for (const StageEntry& e : ents) {
+      Request& r = sv->reqs[sv->slot_req[e.inst]];
+      r.round += 1;
+      if (r.round >= r.num_vars) {
+        SC_HIP(hipMemcpyAsync(r.result, sv->d_states + r.slot, sizeof(ProofState), hipMemcpyDeviceToHost, s));
+        r.state = REQ_FINISHING;
+        rec.finishing.push_back(r.id);
+      }
+    }
+  }
+  if (any) {
+    SC_HIP(hipEventCreateWithFlags(&rec.ev, hipEventDisableTiming));
+    SC_HIP(hipEventRecord(rec.ev, s));
+    sv->inflight.push_back(rec);
+    ++sv->steps;
+  }
+  int running = 0;
+  for (int slot = 0; slot < sv->max_active; ++slot)
+    if (sv->slot_req[slot] != 0) ++running;
+  if (n_running) *n_running = running;
+  if (n_queued) *n_queued = (int32_t)sv->queue.size();
+  return SC_OK;
+}
+
+int sc_server_poll(sc_server* sv, int64_t handle, int32_t* state) {
+  if (sv == nullptr || state == nullptr) return fail(SC_ERR_ARG, "NULL argument");
+  int rc = retire(sv, false);
+  if (rc != SC_OK) return rc;
+  auto it = sv->reqs.find(handle);
+  if (it == sv->reqs.end()) return fail(SC_ERR_ARG, "unknown request handle");
+  *state = it->second.state == REQ_FINISHING ? REQ_RUNNING : it->second.state;
+  return SC_OK;
+}
+
+int sc_server_drain(sc_server* sv) {
+  if (sv == nullptr) return fail(SC_ERR_ARG, "server must not be NULL");
+  return retire(sv, true);
+}
+
+int sc_server_result(sc_server* sv, int64_t handle, uint32_t* round_polys, uint32_t* challenges,
+                     uint64_t* checksum) {
+  if (sv == nullptr) return fail(SC_ERR_ARG, "server must not be NULL");
+  int rc = retire(sv, false);
+  if (rc != SC_OK) return rc;
+  auto it = sv->reqs.find(handle);
+  if (it == sv->reqs.end()) return fail(SC_ERR_ARG, "unknown request handle");
+  Request& r = it->second;
+  if (r.state != REQ_DONE) return fail(SC_ERR_ARG, "request not finished");
+  const ProofState& st = *r.result;
+  uint64_t sum = 0;
+  for (int j = 0; j < r.num_vars; ++j) {
+    for (int t = 0; t < r.npts; ++t) {
+      for (int k = 0; k < r.L; ++k) {
+        const uint32_t w = st.polys[j][t][k];
+        sum += (uint64_t)(k + 1) * w;
+        if (round_polys) round_polys[((size_t)j * r.npts + t) * r.L + k] = w;
+      }
+    }
+    if (challenges && j + 1 < r.num_vars)
+      for (int k = 0; k < r.L; ++k) challenges[(size_t)j * r.L + k] = st.r[j][k];
+  }
+  if (checksum) *checksum = sum;
+  (void)hipHostFree(r.result);
+  sv->reqs.erase(it);
+  return SC_OK;
+}
+
+int sc_server_info(sc_server* sv, uint64_t* arena_used, uint64_t* arena_size, uint64_t* steps) {
+  if (sv == nullptr) return fail(SC_ERR_ARG, "server must not be NULL");
+  if (arena_used) *arena_used = sv->arena.used;
+  if (arena_size) *arena_size = sv->arena.size;
+  if (steps) *steps = sv->steps;
+  return SC_OK;
+}
+
+int sc_server_destroy(sc_server* sv) {
+  if (sv == nullptr) return SC_OK;
+  (void)hipSetDevice(sv->ctx->device);
+  free_server(sv);
   return SC_OK;
 }
 

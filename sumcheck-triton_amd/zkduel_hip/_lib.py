@@ -95,6 +95,16 @@ SIGNATURES = [
     ("sc_batch_result", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, u32p, u32p, u64p]),
     ("sc_batch_info", ctypes.c_int, [ctypes.c_void_p, u64p, i32p, u64p]),
     ("sc_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("sc_server_create", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    ("sc_server_submit", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ScComposition), ctypes.c_int32, ctypes.POINTER(ScInput),
+      ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
+    ("sc_server_step", ctypes.c_int, [ctypes.c_void_p, i32p, i32p]),
+    ("sc_server_poll", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, i32p]),
+    ("sc_server_drain", ctypes.c_int, [ctypes.c_void_p]),
+    ("sc_server_result", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, u32p, u32p, u64p]),
+    ("sc_server_info", ctypes.c_int, [ctypes.c_void_p, u64p, u64p, u64p]),
+    ("sc_server_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("sc_calibrate", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int32]),
     ("sc_transcript_challenge", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, u32p, ctypes.c_int32, u32p]),
