@@ -134,7 +134,7 @@ def measure_plain(ctx, bw: int, comp, nv: int, reps: int, selector_mask: int = 0
 
 
 def measure_batched(ctx, bw: int, n: int = 256, reps: int = 3) -> dict:
-    """Config 4: n uneven instances (num_vars 10..22, random product sums over 2..8 tables),
+    """Batched mix: n uneven instances (num_vars 10..22, random product sums over 2..8 tables),
     proved one at a time (plain) and as one batch; both SHA3. The tile size of the batch is
     picked by timing candidates (autotune) before the timed runs; results must agree."""
     from zkduel_hip import Proof
@@ -183,6 +183,30 @@ def measure_batched(ctx, bw: int, n: int = 256, reps: int = 3) -> dict:
             "batched_proofs_per_s": n / batched_s, "plain_proofs_per_s": n / plain_s,
             "speedup": plain_s / batched_s, "batched_ms": batched_s * 1000.0, "plain_ms": plain_s * 1000.0,
             "results_identical": batch_cs == plain_cs, "device_bytes": info["device_bytes"]}
+
+
+def measure_serving(ctx, rate_per_s: float, n: int, budget_bytes: int = 200 * 10**9) -> dict:
+    """Serving: n requests with Poisson arrivals at rate_per_s (num_vars log-uniform 12..24,
+    widths 256/64/32 at 0.5/0.3/0.2, random product sums) replayed in real time against the
+    continuous-batching server under an HBM budget; every checksum is then checked against
+    the same proof run alone."""
+    from zkduel_hip import Proof
+    from zkduel_hip.serving import run_poisson
+    from zkduel_hip.synthetic import serving_requests
+
+    reqs = serving_requests(n, rate_per_s=rate_per_s, seed=5)
+    st = run_poisson(ctx, reqs, hbm_budget_bytes=budget_bytes, max_active=256)
+    same = True
+    for r in reqs[:: max(1, n // 64)]:  # a spread sample of requests, each re-proved alone
+        with Proof(ctx, r["bit_width"], r["composition"], r["num_vars"], input="uniform",
+                   seed=r["seed"], instance=r["instance"]) as pr:
+            pr.run("sha3")
+            pr.wait()
+            same = same and pr.result()[2] == st["checksums"][r["id"]]
+    out = {k: v for k, v in st.items() if k != "checksums"}
+    out["hbm_budget_bytes"] = budget_bytes
+    out["results_identical_to_alone_sample"] = same
+    return out
 
 
 def cpu_baseline(budget_s: float = 20.0) -> dict:
@@ -235,7 +259,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(instances shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
+              "(independent proofs shard across GPUs with no communication between them)", file=sys.stderr)
         return 2
     if 64 not in args.widths:
         args.widths = [64] + args.widths
@@ -256,17 +280,24 @@ def main() -> int:
     from zkduel_hip import Composition, get_spec
     from zkduel_hip.synthetic import constraint_composition
 
-    log("bench: config 1 (Goldilocks a*b*c, num_vars 16)")
+    log("bench: Goldilocks a*b*c, num_vars 16")
     cfg1 = measure_plain(ctx, 64, Composition.from_spec(get_spec("poly_abc")), 16, max(5, args.steps))
-    log("bench: config 3 (BLS12-381, 10 + 2 selector tables, 12 monomials of degree 3-6, num_vars 22)")
+    log("bench: BLS12-381, 10 + 2 selector tables, 12 monomials of degree 3-6, num_vars 22")
     comp3, mask3 = constraint_composition(256)
     cfg3 = measure_plain(ctx, 256, comp3, 22, 2, selector_mask=mask3)
     batched = {}
     if not args.no_batch:
         for bw in (64, 256):
-            log(f"bench: config 4 batched vs plain, {bw}-bit")
+            log(f"bench: 256 uneven instances batched vs plain, {bw}-bit")
             batched[str(bw)] = measure_batched(ctx, bw)
             log(f"bench:   {batched[str(bw)]['batched_proofs_per_s']:.0f} vs {batched[str(bw)]['plain_proofs_per_s']:.0f} proofs/s")
+    serving = {}
+    if not args.no_batch:
+        for rate, n in ((300.0, 600), (5000.0, 600)):
+            log(f"bench: serving, Poisson arrivals at {rate:.0f}/s, {n} requests")
+            serving[f"rate_{rate:.0f}_per_s"] = measure_serving(ctx, rate, n)
+            r = serving[f"rate_{rate:.0f}_per_s"]
+            log(f"bench:   {r['proofs_per_s']:.0f} proofs/s, p50 {r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms")
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")
@@ -297,10 +328,12 @@ def main() -> int:
                 str(bw): {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
                 for bw, r in per.items()
             },
-            "config1_goldilocks_abc_nv16": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in cfg1.items()},
-            "config3_bls_constraint_nv22": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in cfg3.items()},
-            "config4_batched_vs_plain": {bw: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+            "goldilocks_abc_nv16": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in cfg1.items()},
+            "bls_constraint_12_monomials_nv22": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in cfg3.items()},
+            "batched_256_uneven_vs_plain": {bw: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
                                          for bw, r in batched.items()},
+            "serving_poisson_mixed_widths": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in r.items()}
+                                             for k, r in serving.items()},
             "calibration": {
                 "mad_u64_u32_lane_ops_per_s": calib["mad_u64_u32"],
                 "hbm_read_gbps": calib["hbm_read"] / 1e9,

@@ -1,5 +1,5 @@
 """Serving: a continuous-batching proof server over the library's sc_server, and the
-Poisson-arrival driver of the serving benchmark (config 5).
+Poisson-arrival driver of the serving benchmark.
 
 Requests arrive over time; the server admits them first-in-first-out while a slot and HBM
 budget are free, and every step advances every running request by one round on the device
@@ -94,7 +94,10 @@ def run_poisson(ctx: Context, requests: Sequence[dict], hbm_budget_bytes: int, m
     reqs = sorted(requests, key=lambda r: r["arrival"])
     checksums: dict[int, int] = {}
     latency: dict[int, float] = {}
-    with Server(ctx, hbm_budget_bytes, max_active) as sv:
+    t_setup = time.perf_counter()
+    sv = Server(ctx, hbm_budget_bytes, max_active)
+    setup_s = time.perf_counter() - t_setup
+    try:
         live: dict[int, dict] = {}
         nxt = 0
         t0 = time.perf_counter()
@@ -119,6 +122,10 @@ def run_poisson(ctx: Context, requests: Sequence[dict], hbm_budget_bytes: int, m
                 time.sleep(max(0.0, min(reqs[nxt]["arrival"] - now, 0.001)))
         total = time.perf_counter() - t0
         steps = sv.info()["steps"]
+    finally:
+        t_down = time.perf_counter()
+        sv.close()
+        teardown_s = time.perf_counter() - t_down
     lat = sorted(latency.values())
     return {
         "requests": len(reqs),
@@ -129,5 +136,7 @@ def run_poisson(ctx: Context, requests: Sequence[dict], hbm_budget_bytes: int, m
         "p99_ms": 1000.0 * lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))],
         "peak_running": peak_running,
         "steps": steps,
+        "server_setup_s": setup_s,
+        "server_teardown_s": teardown_s,
         "checksums": checksums,
     }

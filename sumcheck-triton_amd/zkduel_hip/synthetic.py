@@ -1,4 +1,4 @@
-"""Seeded synthetic workloads of the benchmark configurations (compositions and instance mixes).
+"""Seeded synthetic workloads of the benchmarks (compositions, instance mixes, request streams).
 
 All generators are pure functions of their seed (Python's ``random.Random``), so the GPU run,
 the plain-vs-batched comparison and any CPU check see the same instances.
@@ -15,13 +15,13 @@ from .workloads import get_spec
 
 
 def abc_plus_de() -> Composition:
-    """a*b*c + d*e over 5 tables (config 2)."""
+    """a*b*c + d*e over 5 tables (the at-scale plain workload)."""
     return Composition.from_spec(get_spec("poly_abc_plus_de"))
 
 
 def constraint_composition(bit_width: int, seed: int = 3, tables: int = 10, selectors: int = 2,
                            monomials: int = 12) -> tuple[Composition, int]:
-    """High-degree constraint-style composition (config 3): ``tables`` uniform tables plus
+    """High-degree constraint-style composition (BLS12-381, num_vars 22 workload): ``tables`` uniform tables plus
     ``selectors`` selector tables, a sum of ``monomials`` products of degree 3..6 with seeded
     random field constants as coefficients. Every monomial carries one selector, as gate
     constraints do. Returns the composition and the selector mask (the last tables)."""
@@ -41,7 +41,7 @@ def constraint_composition(bit_width: int, seed: int = 3, tables: int = 10, sele
 
 
 def random_product_sum(rng: random.Random) -> Composition:
-    """A random sum of 1..4 products of degree 2..5 over 2..8 tables (config 4)."""
+    """A random sum of 1..4 products of degree 2..5 over 2..8 tables (the batched mix)."""
     T = rng.randrange(2, 9)
     K = rng.randrange(1, 5)
     terms = []
@@ -52,7 +52,7 @@ def random_product_sum(rng: random.Random) -> Composition:
 
 
 def batch_instances(n: int = 256, seed: int = 4, nv_lo: int = 10, nv_hi: int = 22) -> list[tuple]:
-    """Config 4's mix: ``n`` instances, num_vars uniform in [nv_lo, nv_hi], random product sums,
+    """The batched mix: ``n`` instances, num_vars uniform in [nv_lo, nv_hi], random product sums,
     uniform tables keyed by instance number."""
     rng = random.Random(seed)
     out = []
@@ -64,9 +64,9 @@ def batch_instances(n: int = 256, seed: int = 4, nv_lo: int = 10, nv_hi: int = 2
 
 
 def serving_requests(n: int, rate_per_s: float, seed: int = 5, nv_lo: int = 12, nv_hi: int = 24):
-    """Config 5's request stream: Poisson arrivals at ``rate_per_s``, num_vars log-uniform in
+    """The serving request stream: Poisson arrivals at ``rate_per_s``, num_vars log-uniform in
     [nv_lo, nv_hi] (uniform in log2(num_vars)), width 256/64/32 with probability 0.5/0.3/0.2,
-    compositions as in the batched config. Yields dicts with arrival time (s from start)."""
+    compositions as in the batched mix. Yields dicts with arrival time (s from start)."""
     rng = random.Random(seed)
     t = 0.0
     reqs = []

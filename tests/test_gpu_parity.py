@@ -153,7 +153,7 @@ def test_round_polys_match_oracle_uniform_sha3(ctx, bw, num_vars):
 
 @pytest.mark.parametrize("bw", WIDTHS)
 def test_random_field_coefficients_and_selectors(ctx, bw):
-    """cfg3-style compositions: random field constants, selector tables in {0,1}, degree 3..6."""
+    """Constraint-style compositions: random field constants, selector tables in {0,1}, degree 3..6."""
     f = get_field(bw)
     rng = random.Random(500 + bw)
     T = 12

@@ -66,8 +66,8 @@ def test_server_results_match_alone(ctx):
 
 def test_server_budget_admits_one_at_a_time(ctx):
     comp = Composition.from_spec(get_spec("poly_abc_plus_de"))
-    nv = 12  # 5 tables x 4096 x 8 B = 160 KiB input, 280 KiB with fold buffers
-    with Server(ctx, 400 << 10, max_active=16) as sv:
+    nv = 14  # 5 tables x 16384 x 8 B = 640 KiB input, 1.12 MiB with the fold buffers
+    with Server(ctx, 3 << 19, max_active=16) as sv:
         hs = [sv.submit(64, comp, nv, {"seed": 3, "instance": i}) for i in range(5)]
         running, queued = sv.step()
         assert running == 1 and queued == 4
