@@ -134,7 +134,9 @@ int sc_prove(sc_context* ctx, int32_t bit_width, const sc_composition* comp, int
 /* Elementwise device field operations on n elements (host buffers, [n][L]):
  * op 0: a*b*R^-1 (Montgomery product), 1: a+b, 2: a-b, 3: a -> a*R (encode),
  * 4: a -> a*R^-1 (decode). The device twin of src/zkduel/fields.py:63-115, and what the
- * reference's tests/test_uint256.py exercises. b may be NULL for ops 3 and 4. */
+ * reference's tests/test_uint256.py exercises. b may be NULL for ops 3 and 4.
+ * ops 5 and 6 run the prover kernels' internal arithmetic path on Montgomery inputs and return
+ * Montgomery outputs: 5: a*b*R^-1 (as op 0), 6: a + a*(b - a)*R^-1 (a fold with r = a). */
 int sc_field_op(sc_context* ctx, int32_t bit_width, int32_t op, uint64_t n, const uint32_t* a,
                 const uint32_t* b, uint32_t* out);
 

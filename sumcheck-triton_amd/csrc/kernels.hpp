@@ -19,12 +19,38 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "field.hpp"
 #include "keccak.hpp"
 #include "layout.hpp"
 #include "prng.hpp"
+#include "work.hpp"
 
 namespace sc {
+
+// fn(integral_constant<int, I>) for I = 0..N-1, expanded at compile time. Used where a loop
+// body is large enough that the compiler might keep the loop and index register arrays
+// through scratch memory.
+template <int N, int I = 0, class Fn>
+__device__ __forceinline__ void static_for(Fn&& fn) {
+  if constexpr (I < N) {
+    fn(std::integral_constant<int, I>{});
+    static_for<N, I + 1>(static_cast<Fn&&>(fn));
+  }
+}
+
+// static_for when EXPAND, else an ordinary loop left to the compiler's unroller (for the
+// interpreted kernels with up to 32 tables, whose full expansion is too large to compile).
+template <int N, bool EXPAND, class Fn>
+__device__ __forceinline__ void table_for(Fn&& fn) {
+  if constexpr (EXPAND) {
+    static_for<N>(static_cast<Fn&&>(fn));
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; ++i) fn(i);
+  }
+}
 
 // ---------------------------------------------------------------------------------------------
 // Vector memory helpers (element-major: NW consecutive words per thread)
@@ -84,18 +110,18 @@ __device__ __forceinline__ Elem<L> load_elem(const uint32_t* p) {
 // C(x) = sum_k c_k prod_v x_v^e_kv. Term structure is wave-uniform, so every branch below is
 // a scalar branch; table slots are indexed statically (unrolled v), keeping x in VGPRs.
 template <class F, int TMAX>
-__device__ __forceinline__ void compose_add(Accum<F>& acc, const Elem<F::L> (&x)[TMAX],
+__device__ __forceinline__ void compose_add(typename Work<F>::Acc& acc, const typename Work<F>::V (&x)[TMAX],
                                             const CompDesc* __restrict__ cd, int K) {
-  constexpr int L = F::L;
+  using W = Work<F>;
   for (int k = 0; k < K; ++k) {
-    Elem<L> prod = zero<F>();
+    typename W::V prod = x[0];  // every term has a factor, so prod is set before it is used
     bool have = false;
 #pragma unroll
     for (int v = 0; v < TMAX; ++v) {
       uint32_t e = (cd->exps[k][v >> 3] >> ((v & 7) * 4)) & 15u;
       for (; e != 0u; --e) {
         if (have) {
-          prod = mont_mul<F>(prod, x[v]);
+          prod = W::mul(prod, x[v]);
         } else {
           prod = x[v];
           have = true;
@@ -108,7 +134,7 @@ __device__ __forceinline__ void compose_add(Accum<F>& acc, const Elem<F::L> (&x)
     } else if (kind == COEFF_MINUS_ONE) {
       acc.sub(prod);
     } else {
-      acc.add(mont_mul<F>(prod, load_elem<L>(cd->coeff[k])));
+      acc.add(W::mul(prod, W::coeff(cd->coeff[k])));
     }
   }
 }
@@ -116,27 +142,30 @@ __device__ __forceinline__ void compose_add(Accum<F>& acc, const Elem<F::L> (&x)
 // Compiled composition STATIC_COMPS[SID]: every loop has constant bounds and indices, so it
 // unrolls into straight-line field operations on fixed registers.
 template <class F, int SID, int TMAX>
-__device__ __forceinline__ void compose_static(Accum<F>& acc, const Elem<F::L> (&x)[TMAX],
+__device__ __forceinline__ void compose_static(typename Work<F>::Acc& acc, const typename Work<F>::V (&x)[TMAX],
                                                const CompDesc* __restrict__ cd) {
-  constexpr StaticComp C = STATIC_COMPS[SID];
-  static_assert(C.num_tables <= TMAX, "table slots");
-#pragma unroll
-  for (int k = 0; k < C.num_terms; ++k) {
-    Elem<F::L> prod = x[C.vars[k][0]];
-#pragma unroll
-    for (int m = 1; m < C.lens[k]; ++m) prod = mont_mul<F>(prod, x[C.vars[k][m]]);
-    if (C.kinds[k] == COEFF_ONE) {
+  using W = Work<F>;
+  static_assert(STATIC_COMPS[SID].num_tables <= TMAX, "table slots");
+  static_for<STATIC_COMPS[SID].num_terms>([&](auto ki) {
+    constexpr int k = decltype(ki)::value;
+    constexpr StaticComp C = STATIC_COMPS[SID];
+    typename W::V prod = x[C.vars[k][0]];
+    static_for<C.lens[k] - 1>([&](auto mi) {
+      constexpr int m = decltype(mi)::value + 1;
+      prod = W::mul(prod, x[STATIC_COMPS[SID].vars[k][m]]);
+    });
+    if constexpr (C.kinds[k] == COEFF_ONE) {
       acc.add(prod);
-    } else if (C.kinds[k] == COEFF_MINUS_ONE) {
+    } else if constexpr (C.kinds[k] == COEFF_MINUS_ONE) {
       acc.sub(prod);
     } else {
-      acc.add(mont_mul<F>(prod, load_elem<F::L>(cd->coeff[k])));
+      acc.add(W::mul(prod, W::coeff(cd->coeff[k])));
     }
-  }
+  });
 }
 
 template <class F, int SID, int TMAX>
-__device__ __forceinline__ void compose(Accum<F>& acc, const Elem<F::L> (&x)[TMAX],
+__device__ __forceinline__ void compose(typename Work<F>::Acc& acc, const typename Work<F>::V (&x)[TMAX],
                                         const CompDesc* __restrict__ cd, int K) {
   if constexpr (SID == 0) {
     compose_add<F, TMAX>(acc, x, cd, K);
@@ -164,67 +193,69 @@ struct Shape {
 // addition of the difference (reference.py:89-99); those sums may stay unreduced where the
 // field allows (step_add).
 template <class F, int TMAX, int PMAX, int SID>
-__device__ __forceinline__ void eval_points(Elem<F::L> (&cur)[TMAX], Elem<F::L> (&oth)[TMAX],
-                                            Accum<F> (&acc)[PMAX], const CompDesc* __restrict__ cd,
+__device__ __forceinline__ void eval_points(typename Work<F>::V (&cur)[TMAX], typename Work<F>::V (&oth)[TMAX],
+                                            typename Work<F>::Acc (&acc)[PMAX], const CompDesc* __restrict__ cd,
                                             int T, int K, int npts) {
+  using W = Work<F>;
+  constexpr bool X = SID != 0;  // compiled compositions expand every loop
   compose<F, SID, TMAX>(acc[0], cur, cd, K);
-#pragma unroll
-  for (int v = 0; v < TMAX; ++v) {
+  table_for<TMAX, X>([&](auto vi) {
+    const int v = vi;
     if (v < T) {
-      const Elem<F::L> o = oth[v];
-      oth[v] = sub<F>(o, cur[v]);
+      const typename W::V o = oth[v];
+      oth[v] = W::diff(o, cur[v]);
       cur[v] = o;
     }
-  }
+  });
   compose<F, SID, TMAX>(acc[1], cur, cd, K);
-#pragma unroll
-  for (int t = 2; t < PMAX; ++t) {
+  table_for<PMAX - 2, X>([&](auto ti) {
+    const int t = (int)ti + 2;
     if (t < npts) {
-#pragma unroll
-      for (int v = 0; v < TMAX; ++v) {
-        if (v < T) cur[v] = step_add<F>(cur[v], oth[v]);
-      }
+      table_for<TMAX, X>([&](auto vi) {
+        const int v = vi;
+        if (v < T) cur[v] = W::step(cur[v], oth[v]);
+      });
       compose<F, SID, TMAX>(acc[t], cur, cd, K);
     }
-  }
+  });
 }
 
 // Loads pair i of the current round for every table, folding from the previous round's
 // tables when fold != 0 (f' = e + r (o - e), field_sumcheck.cu:677-694), and writes the
-// folded entries to dst.
+// folded entries to dst. src holds the caller's Montgomery tables when src_mont (rounds 0
+// and 1), else entries written by an earlier fold.
 template <class F, int TMAX>
-__device__ __forceinline__ void load_pair(Elem<F::L> (&cur)[TMAX], Elem<F::L> (&oth)[TMAX],
+__device__ __forceinline__ void load_pair(typename Work<F>::V (&cur)[TMAX], typename Work<F>::V (&oth)[TMAX],
                                           const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
-                                          uint64_t src_n, uint64_t i, int T, bool fold,
-                                          const Elem<F::L>& r) {
+                                          uint64_t src_n, uint64_t i, int T, bool fold, bool src_mont,
+                                          const typename Work<F>::V& r) {
+  using W = Work<F>;
   constexpr int L = F::L;
-#pragma unroll
-  for (int v = 0; v < TMAX; ++v) {
+  table_for<TMAX, (TMAX <= 8)>([&](auto vi) {
+    const int v = vi;
     if (v < T) {
       if (fold) {
         uint32_t w[4 * L];
         load_words<4 * L>(w, src + ((uint64_t)v * src_n + 4 * i) * L);
-        const Elem<L> f0 = elem_at<L>(w, 0), f1 = elem_at<L>(w, 1);
-        const Elem<L> f2 = elem_at<L>(w, 2), f3 = elem_at<L>(w, 3);
-        const Elem<L> e = add<F>(f0, mont_mul<F>(r, sub<F>(f1, f0)));
-        const Elem<L> o = add<F>(f2, mont_mul<F>(r, sub<F>(f3, f2)));
-        uint32_t y[2 * L];
+        typename W::V f[4];
 #pragma unroll
-        for (int k = 0; k < L; ++k) {
-          y[k] = e.v[k];
-          y[L + k] = o.v[k];
-        }
+        for (int k = 0; k < 4; ++k) f[k] = src_mont ? W::from_mont(w + k * L) : W::from_stored(w + k * L);
+        const typename W::V e = W::fold(f[0], f[1], r);
+        const typename W::V o = W::fold(f[2], f[3], r);
+        uint32_t y[2 * L];
+        W::store(y, e);
+        W::store(y + L, o);
         store_words<2 * L>(dst + ((uint64_t)v * (src_n >> 1) + 2 * i) * L, y);
         cur[v] = e;
         oth[v] = o;
       } else {
         uint32_t w[2 * L];
         load_words<2 * L>(w, src + ((uint64_t)v * src_n + 2 * i) * L);
-        cur[v] = elem_at<L>(w, 0);
-        oth[v] = elem_at<L>(w, 1);
+        cur[v] = W::from_mont(w);
+        oth[v] = W::from_mont(w + L);
       }
     }
-  }
+  });
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -374,16 +405,17 @@ __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
   const int npts = Shape<SID>::npts(cd);
   const bool fold = a.fold != 0;
 
-  Accum<F> acc[PMAX];
+  using W = Work<F>;
+  typename W::Acc acc[PMAX];
 #pragma unroll
   for (int t = 0; t < PMAX; ++t) acc[t].clear();
-  Elem<L> r = zero<F>();
-  if (fold) r = load_elem<L>(a.st->r[a.round - 1]);
+  typename W::V r = W::from_mont(a.st->r[fold ? a.round - 1 : 0]);  // unused in round 0
+  const bool src_mont = a.round <= 1;
 
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.pairs; i += stride) {
-    Elem<L> cur[TMAX], oth[TMAX];
-    load_pair<F, TMAX>(cur, oth, a.src, a.dst, a.src_n, i, T, fold, r);
+    typename W::V cur[TMAX], oth[TMAX];
+    load_pair<F, TMAX>(cur, oth, a.src, a.dst, a.src_n, i, T, fold, src_mont, r);
     eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts);
   }
   Elem<L> sums[PMAX];
@@ -435,7 +467,7 @@ __global__ void __launch_bounds__(TB) tail_kernel(TailArgs a) {
   const int K = Shape<SID>::terms(cd);
   const int npts = Shape<SID>::npts(cd);
 
-  if (a.first_round > 0 && threadIdx.x < L) s_r[threadIdx.x] = a.st->r[a.first_round - 1][threadIdx.x];
+  if (threadIdx.x < L) s_r[threadIdx.x] = a.first_round > 0 ? a.st->r[a.first_round - 1][threadIdx.x] : 0u;
   __syncthreads();
   for (int j = a.first_round; j < a.num_rounds; ++j) {
     const bool fold = j > 0;
@@ -443,13 +475,14 @@ __global__ void __launch_bounds__(TB) tail_kernel(TailArgs a) {
     uint32_t* dst = (j & 1) ? a.buf_a : a.buf_b;
     const uint64_t src_n = fold ? (a.n0 >> (j - 1)) : a.n0;
     const uint64_t pairs = a.n0 >> (j + 1);
-    const Elem<L> r = load_elem<L>(s_r);
-    Accum<F> acc[PMAX];
+    using W = Work<F>;
+    const typename W::V r = W::from_mont(s_r);
+    typename W::Acc acc[PMAX];
 #pragma unroll
     for (int t = 0; t < PMAX; ++t) acc[t].clear();
     for (uint64_t i = threadIdx.x; i < pairs; i += TB) {
-      Elem<L> cur[TMAX], oth[TMAX];
-      load_pair<F, TMAX>(cur, oth, src, dst, src_n, i, T, fold, r);
+      typename W::V cur[TMAX], oth[TMAX];
+      load_pair<F, TMAX>(cur, oth, src, dst, src_n, i, T, fold, j <= 1, r);
       eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts);
     }
     Elem<L> sums[PMAX];
@@ -495,10 +528,10 @@ __global__ void __launch_bounds__(BLOCK) batch_pass_kernel(BatchArgs a) {
   uint32_t* dst = (j & 1) ? in.buf_a : in.buf_b;
   const uint64_t src_n = fold ? (in.n0 >> (j - 1)) : in.n0;
   const uint64_t pairs = in.n0 >> (j + 1);
-  Elem<L> r = zero<F>();
-  if (fold) r = load_elem<L>(in.st->r[j - 1]);
+  using W = Work<F>;
+  const typename W::V r = W::from_mont(in.st->r[fold ? j - 1 : 0]);  // unused in round 0
 
-  Accum<F> acc[PMAX];
+  typename W::Acc acc[PMAX];
 #pragma unroll
   for (int t = 0; t < PMAX; ++t) acc[t].clear();
   const uint64_t tile = blockIdx.x - ent.tile_begin;
@@ -506,8 +539,8 @@ __global__ void __launch_bounds__(BLOCK) batch_pass_kernel(BatchArgs a) {
   for (uint32_t it = 0; it < a.items; ++it) {
     const uint64_t i = base + (uint64_t)it * BLOCK + threadIdx.x;
     if (i < pairs) {
-      Elem<L> cur[TMAX], oth[TMAX];
-      load_pair<F, TMAX>(cur, oth, src, dst, src_n, i, T, fold, r);
+      typename W::V cur[TMAX], oth[TMAX];
+      load_pair<F, TMAX>(cur, oth, src, dst, src_n, i, T, fold, j <= 1, r);
       eval_points<F, TMAX, PMAX, 0>(cur, oth, acc, cd, T, K, npts);
     }
   }
@@ -572,7 +605,11 @@ __global__ void __launch_bounds__(BLOCK) fill_kernel(FillArgs a) {
 }
 
 // Elementwise field operations, for the field-layer tests and host data conversion.
-enum FieldOp : int32_t { OP_MUL = 0, OP_ADD = 1, OP_SUB = 2, OP_TO_MONT = 3, OP_FROM_MONT = 4 };
+// OP_WORK_MUL and OP_WORK_FOLD run the kernels' working-form path end to end: a*b, and
+// a + a (b - a) through fold, store and reload, each read back through the accumulator.
+enum FieldOp : int32_t {
+  OP_MUL = 0, OP_ADD = 1, OP_SUB = 2, OP_TO_MONT = 3, OP_FROM_MONT = 4, OP_WORK_MUL = 5, OP_WORK_FOLD = 6
+};
 
 template <class F>
 __global__ void __launch_bounds__(BLOCK) field_op_kernel(int op, uint64_t n, const uint32_t* a,
@@ -590,6 +627,19 @@ __global__ void __launch_bounds__(BLOCK) field_op_kernel(int op, uint64_t n, con
       y = sub<F>(x, load_elem<L>(b + i * L));
     } else if (op == OP_TO_MONT) {
       y = mont_mul<F>(x, load_elem<L>(fc->r2));
+    } else if (op == OP_WORK_MUL || op == OP_WORK_FOLD) {
+      using W = Work<F>;
+      const typename W::V wa = W::from_mont(a + i * L), wb = W::from_mont(b + i * L);
+      typename W::Acc acc;
+      acc.clear();
+      if (op == OP_WORK_MUL) {
+        acc.add(W::mul(wa, wb));
+      } else {
+        uint32_t w[L];
+        W::store(w, W::fold(wa, wb, wa));
+        acc.add(W::from_stored(w));
+      }
+      y = acc.value();
     } else {
       Elem<L> one = zero<F>();
       one.v[0] = 1u;

@@ -21,6 +21,7 @@
 #include "field.hpp"
 #include "launch.hpp"
 #include "layout.hpp"
+#include "work.hpp"
 
 using namespace sc;
 
@@ -141,7 +142,7 @@ int build_comp(const sc_composition* c, const FieldConsts& fc, CompDesc& d, int&
     } else {
       d.kind[k] = COEFF_GENERAL;
     }
-    const Elem<L> cm = H::to_mont(coeff, fc);
+    const Elem<L> cm = work_coeff<F>(H::to_mont(coeff, fc));
     for (int i = 0; i < L; ++i) d.coeff[k][i] = cm.v[i];
   }
   d.npts = degree + 1;
@@ -708,8 +709,8 @@ int sc_field_op(sc_context* ctx, int32_t bit_width, int32_t op, uint64_t n, cons
   if (ctx == nullptr || a == nullptr || out == nullptr) return fail(SC_ERR_ARG, "NULL argument");
   const int fidx = field_index(bit_width);
   if (fidx < 0) return fail(SC_ERR_ARG, "bit_width must be 32, 64, 128 or 256");
-  if (op < 0 || op > 4) return fail(SC_ERR_ARG, "op must be in [0, 4]");
-  if (op <= 2 && b == nullptr) return fail(SC_ERR_ARG, "b must not be NULL for this op");
+  if (op < 0 || op > 6) return fail(SC_ERR_ARG, "op must be in [0, 6]");
+  if ((op <= 2 || op >= 5) && b == nullptr) return fail(SC_ERR_ARG, "b must not be NULL for this op");
   if (n == 0) return SC_OK;
   SC_HIP(hipSetDevice(ctx->device));
   const Launchers& ln = launchers_for(fidx);

@@ -1,0 +1,313 @@
+// Working form of field elements inside the prover kernels.
+//
+// Tables arrive as Montgomery limbs in [0, p) (R = 2^(32L)), and round polynomials and
+// challenges leave in that form. In between, a kernel may hold values in any representation
+// that maps back to the same residues; `Work<F>` names the one each field uses:
+//
+//   from_mont(w)      caller's Montgomery limbs -> working value
+//   from_stored(w)    fold-buffer words (written by store) -> working value
+//   store(w, x)       working value -> fold-buffer words (same size as a Montgomery element)
+//   coeff(w)          composition coefficient, already in working form on the host (work_coeff)
+//   mul(a, b)         product
+//   diff(hi, lo)      hi - lo, the interpolation step of a pair
+//   step(x, d)        x + d
+//   fold(lo, hi, r)   lo + r (hi - lo), the folded entry, ready for store
+//   Acc               running sum: add, sub, value() -> canonical Montgomery limbs
+//
+// The 2^32-5, Goldilocks and 2^127-1 fields work on their Montgomery limbs directly. The
+// BLS12-381 scalar field works in radix 2^29 (R29 below): a 32x32-bit multiply-add then never
+// overflows a 64-bit column sum, so Montgomery multiplication needs no carry chains.
+#pragma once
+
+#include "field.hpp"
+
+namespace sc {
+
+template <class F>
+struct Work {
+  static constexpr int L = F::L;
+  using V = Elem<L>;
+  SC_HD static V load(const uint32_t* w) {
+    V x;
+#pragma unroll
+    for (int i = 0; i < L; ++i) x.v[i] = w[i];
+    return x;
+  }
+  SC_HD static V from_mont(const uint32_t* w) { return load(w); }
+  SC_HD static V from_stored(const uint32_t* w) { return load(w); }
+  SC_HD static V coeff(const uint32_t* w) { return load(w); }
+  SC_HD static void store(uint32_t* w, const V& x) {
+#pragma unroll
+    for (int i = 0; i < L; ++i) w[i] = x.v[i];
+  }
+  SC_HD static V mul(const V& a, const V& b) { return mont_mul<F>(a, b); }
+  SC_HD static V diff(const V& hi, const V& lo) { return sub<F>(hi, lo); }
+  SC_HD static V step(const V& x, const V& d) { return step_add<F>(x, d); }
+  SC_HD static V fold(const V& lo, const V& hi, const V& r) { return add<F>(lo, mont_mul<F>(r, sub<F>(hi, lo))); }
+  using Acc = Accum<F>;
+};
+
+// Host: Montgomery coefficient -> the word form Work<F>::coeff reads.
+template <class F>
+inline Elem<F::L> work_coeff(const Elem<F::L>& c) {
+  return c;
+}
+
+// ---------------------------------------------------------------------------------------------
+// R29: BLS12-381 scalar field in nine 29-bit limbs, Montgomery radix R' = 2^261.
+//
+// A residue x is held as any value congruent to x * 2^261 (mod p), limbs little-endian; the
+// Montgomery image X = x * 2^256 of the caller is 32 X in this form, so conversion in is a
+// shifted unpacking and conversion out divides by 32 mod p. Values are not kept in [0, p):
+// the bounds each operation needs are stated with it and hold for every use in the kernels
+// (x(t) < 29p for t <= 7, products < 13p, sums reduced before they can overflow).
+//   mul:   a, b with limbs < 2^31, one of them < 2^29 (or both < 2^30): a column sum of 9
+//          products of a*b, 8 of m*p, m_k and the carry stays below 2^63.5. Result limbs
+//          0..7 < 2^29, value < a*b/2^261 + p.
+//   reduce: any limbs < 2^30, top limb < 2^32 -> value < p + 2^241 (< 2^256, so it packs
+//          into eight 32-bit words), normalized.
+// ---------------------------------------------------------------------------------------------
+struct R29 {
+  uint32_t l[9];
+};
+
+namespace r29 {
+constexpr uint32_t MASK = (1u << 29) - 1;
+
+SC_HD constexpr uint32_t P(int i) {  // p in radix 2^29
+  return i == 0   ? 0x00000001u
+         : i == 1 ? 0x1FFFFFF8u
+         : i == 2 ? 0x1F96FFBFu
+         : i == 3 ? 0x1B4805FFu
+         : i == 4 ? 0x1D80553Bu
+         : i == 5 ? 0x0C0404D0u
+         : i == 6 ? 0x1520CCE7u
+         : i == 7 ? 0x0A6533AFu
+                  : 0x0073EDA7u;
+}
+
+SC_HD constexpr uint32_t NP(int i) {  // 2^261 - p in radix 2^29
+  return i == 0   ? 0x1FFFFFFFu
+         : i == 1 ? 0x00000007u
+         : i == 2 ? 0x00690040u
+         : i == 3 ? 0x04B7FA00u
+         : i == 4 ? 0x027FAAC4u
+         : i == 5 ? 0x13FBFB2Fu
+         : i == 6 ? 0x0ADF3318u
+         : i == 7 ? 0x159ACC50u
+                  : 0x1F8C1258u;
+}
+
+// limb i of k*p in radix 2^29 (k*p < 2^261 for k <= 70)
+SC_HD constexpr uint32_t KP(uint32_t k, int i) {
+  uint64_t c = 0;
+  uint32_t out = 0;
+  for (int j = 0; j <= i; ++j) {
+    c += (uint64_t)k * P(j);
+    out = (uint32_t)(c & MASK);
+    c >>= 29;
+  }
+  return i == 8 ? out + (uint32_t)(c << 29) : out;
+}
+
+// k*p written with every limb in [2^29 - 1, 2^30): limb i lends 1 to limb i-1, so subtracting
+// any normalized b with value(b) < (k-1) p limb by limb leaves no negative limb.
+SC_HD constexpr uint32_t PAD(uint32_t k, int i) {
+  return i == 0 ? KP(k, 0) + (1u << 29) : (i < 8 ? KP(k, i) + MASK : KP(k, 8) - 1u);
+}
+
+// q = floor(t / (p_top + 1)) for t < 2^32, p_top = p >> 232 = 0x73EDA7 (exact multiply-high).
+SC_HD uint32_t top_quotient(uint32_t t) {
+  return (uint32_t)(((uint64_t)t * 0x8D542469u) >> 54);
+}
+}  // namespace r29
+
+SC_HD R29 r29_zero() {
+  R29 z;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) z.l[i] = 0u;
+  return z;
+}
+
+// Carries of limbs 0..7 moved up; the top limb keeps the excess.
+SC_HD R29 r29_normalize(R29 x) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    x.l[i + 1] += x.l[i] >> 29;
+    x.l[i] &= r29::MASK;
+  }
+  return x;
+}
+
+// x - q p with q from the top limb, computed as x + q (2^261 - p) with the q * 2^261 dropped.
+SC_HD R29 r29_reduce(const R29& x) {
+  const uint32_t q = r29::top_quotient(x.l[8]);
+  R29 r;
+  uint64_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    c += (uint64_t)q * r29::NP(i) + x.l[i];
+    r.l[i] = (uint32_t)c & r29::MASK;
+    c >>= 29;
+  }
+  return r;
+}
+
+// Montgomery product a * b * 2^-261 mod p by product scanning (column k of a*b and of m*p
+// summed in 64 bits; m_k clears the column's low 29 bits). p = 1 (mod 2^29), so
+// -p^-1 = -1 and m_k = -column mod 2^29.
+SC_HD R29 r29_mul(const R29& a, const R29& b) {
+  uint32_t m[9];
+  R29 r;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    uint64_t s1 = acc + (uint64_t)a.l[k] * b.l[0];
+    uint64_t s2 = 0;
+#pragma unroll
+    for (int i = 0; i < k; ++i) {
+      s1 += (uint64_t)a.l[i] * b.l[k - i];
+      s2 += (uint64_t)m[i] * r29::P(k - i);
+    }
+    uint64_t s = s1 + s2;
+    m[k] = (0u - (uint32_t)s) & r29::MASK;
+    s += m[k];
+    acc = s >> 29;
+  }
+#pragma unroll
+  for (int k = 9; k < 17; ++k) {
+    uint64_t s1 = acc;
+    uint64_t s2 = 0;
+#pragma unroll
+    for (int i = k - 8; i < 9; ++i) {
+      s1 += (uint64_t)a.l[i] * b.l[k - i];
+      s2 += (uint64_t)m[i] * r29::P(k - i);
+    }
+    const uint64_t s = s1 + s2;
+    r.l[k - 9] = (uint32_t)s & r29::MASK;
+    acc = s >> 29;
+  }
+  r.l[8] = (uint32_t)acc;
+  return r;
+}
+
+// Eight 32-bit words W -> limbs of W * 2^S (S = 0: fold buffers; S = 5: Montgomery input,
+// turning X = x 2^256 into 32 X = x 2^261).
+template <int S>
+SC_HD R29 r29_unpack(const uint32_t* w) {
+  R29 r;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int lo = 29 * k - S;
+    uint32_t x;
+    if (lo < 0) {
+      x = w[0] << S;
+    } else {
+      const int q = lo >> 5, b = lo & 31;
+      x = w[q] >> b;
+      if (b > 3 && q + 1 < 8) x |= w[q + 1] << (32 - b);
+    }
+    r.l[k] = x & r29::MASK;
+  }
+  return r;
+}
+
+// Bits [S, S + 256) of a normalized value as eight 32-bit words.
+template <int S>
+SC_HD void r29_pack(uint32_t* w, const R29& x) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int lo = 32 * j + S;
+    const int k = lo / 29, b = lo % 29;
+    uint32_t y = x.l[k] >> b;
+    if (k + 1 < 9) y |= x.l[k + 1] << (29 - b);
+    if (k + 2 < 9 && 58 - b < 32) y |= x.l[k + 2] << (58 - b);
+    w[j] = y;
+  }
+}
+
+// x * 2^-5 mod p in [0, p) as Montgomery limbs (x < 2p, normalized): x + k p with
+// k = -x mod 32 (p = 1 mod 32) is divisible by 32, and (x + k p) / 32 < 1.04 p.
+SC_HD Elem<8> r29_to_mont(const R29& x) {
+  const uint32_t k = (0u - x.l[0]) & 31u;
+  R29 u;
+  uint64_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    c += (uint64_t)k * r29::P(i) + x.l[i];
+    u.l[i] = (uint32_t)c & r29::MASK;
+    c >>= 29;
+  }
+  Elem<8> y;
+  r29_pack<5>(y.v, u);
+  Elem<8> t;
+  const uint32_t borrow = sub_p<FieldP256>(t, y);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) y.v[i] = borrow ? y.v[i] : t.v[i];
+  return y;
+}
+
+template <>
+struct Work<FieldP256> {
+  static constexpr int L = 8;
+  using V = R29;
+  SC_HD static V from_mont(const uint32_t* w) { return r29_reduce(r29_unpack<5>(w)); }
+  SC_HD static V from_stored(const uint32_t* w) { return r29_unpack<0>(w); }
+  SC_HD static V coeff(const uint32_t* w) { return r29_unpack<0>(w); }
+  SC_HD static void store(uint32_t* w, const V& x) { r29_pack<0>(w, x); }
+  SC_HD static V mul(const V& a, const V& b) { return r29_mul(a, b); }
+  // hi + 3p - lo for hi, lo < p + 2^241: value < 4p + 2^241.
+  SC_HD static V diff(const V& hi, const V& lo) {
+    R29 d;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) d.l[i] = hi.l[i] + r29::PAD(3, i) - lo.l[i];
+    return r29_normalize(d);
+  }
+  SC_HD static V step(const V& x, const V& d) {
+    R29 s;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) s.l[i] = x.l[i] + d.l[i];
+    return r29_normalize(s);
+  }
+  // lo + r (hi + 3p - lo), reduced below p + 2^241. The difference is passed to the product
+  // with limbs < 2^31 (r's are < 2^29), and the sum's carries are resolved by the reduction.
+  SC_HD static V fold(const V& lo, const V& hi, const V& r) {
+    R29 d;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) d.l[i] = hi.l[i] + r29::PAD(3, i) - lo.l[i];
+    const R29 m = r29_mul(r, d);
+    R29 s;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) s.l[i] = lo.l[i] + m.l[i];
+    return r29_reduce(s);
+  }
+  // Running sum of products (each < 16p): limbs 0..7 normalized, the top limb reduced
+  // before it can reach 2^32.
+  struct Acc {
+    R29 v;
+    SC_HD void clear() { v = r29_zero(); }
+    SC_HD void add(const R29& x) {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) v.l[i] += x.l[i];
+      v = r29_normalize(v);
+      if (v.l[8] >= (1u << 31)) v = r29_reduce(v);
+    }
+    SC_HD void sub(const R29& x) {  // adds 17p - x
+#pragma unroll
+      for (int i = 0; i < 9; ++i) v.l[i] += r29::PAD(17, i) - x.l[i];
+      v = r29_normalize(v);
+      if (v.l[8] >= (1u << 31)) v = r29_reduce(v);
+    }
+    SC_HD Elem<8> value() const { return r29_to_mont(r29_reduce(v)); }
+  };
+};
+
+template <>
+inline Elem<8> work_coeff<FieldP256>(const Elem<8>& c) {
+  // 32 c mod p: the R29 image of the Montgomery value c, below p so it packs.
+  Elem<8> x = c;
+  for (int i = 0; i < 5; ++i) x = add<FieldP256>(x, x);
+  return x;
+}
+
+}  // namespace sc

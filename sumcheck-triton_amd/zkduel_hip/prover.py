@@ -117,7 +117,7 @@ class Context:
 
     def field_op(self, bit_width: int, op: str, a: np.ndarray, b: np.ndarray | None = None) -> np.ndarray:
         """Elementwise device op on [n][L] limb arrays: mul (Montgomery), add, sub, to_mont, from_mont."""
-        ops = {"mul": 0, "add": 1, "sub": 2, "to_mont": 3, "from_mont": 4}
+        ops = {"mul": 0, "add": 1, "sub": 2, "to_mont": 3, "from_mont": 4, "work_mul": 5, "work_fold": 6}
         f = get_field(bit_width)
         a = np.ascontiguousarray(a, dtype=np.uint32).reshape(-1, f.limbs)
         bb = None if b is None else np.ascontiguousarray(b, dtype=np.uint32).reshape(-1, f.limbs)

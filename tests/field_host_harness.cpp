@@ -3,6 +3,7 @@
 #include <stdint.h>
 
 #include "field.hpp"
+#include "work.hpp"
 
 using namespace sc;
 
@@ -31,5 +32,75 @@ extern "C" void field_host_op(int bit_width, int op, uint64_t n, const uint32_t*
     case 64: run<FieldP64>(op, n, a, b, out); break;
     case 128: run<FieldP128>(op, n, a, b, out); break;
     default: run<FieldP256>(op, n, a, b, out); break;
+  }
+}
+
+// Working-form path of the prover kernels (csrc/work.hpp), on Montgomery inputs a, b, c:
+//   op 0: a*b                         op 1: a + c (b - a), through fold, store and reload
+//   op 2: -(c * x^3) with x = a + 7 (b - a) reached by seven interpolation steps
+//   op 3: sum of 40 products a*b minus one, through the accumulator (39 a*b)
+//   op 4: a*b - c*x(t) for every t in 0..7, summed (x(t) = a + t (b - a))
+template <class F>
+static void run_work(int op, uint64_t n, const uint32_t* a, const uint32_t* b, const uint32_t* c, uint32_t* out) {
+  using W = Work<F>;
+  constexpr int L = F::L;
+  for (uint64_t i = 0; i < n; ++i) {
+    const typename W::V x = W::from_mont(a + i * L), y = W::from_mont(b + i * L), z = W::from_mont(c + i * L);
+    typename W::Acc acc;
+    acc.clear();
+    if (op == 0) {
+      acc.add(W::mul(x, y));
+    } else if (op == 1) {
+      uint32_t w[L];
+      W::store(w, W::fold(x, y, z));
+      acc.add(W::from_stored(w));
+    } else if (op == 2) {
+      const typename W::V d = W::diff(y, x);
+      typename W::V t = y;
+      for (int k = 2; k <= 7; ++k) t = W::step(t, d);
+      acc.sub(W::mul(W::mul(W::mul(t, t), t), z));
+    } else if (op == 3) {
+      const typename W::V pr = W::mul(x, y);
+      for (int k = 0; k < 40; ++k) acc.add(pr);
+      acc.sub(pr);
+    } else {
+      const typename W::V d = W::diff(y, x);
+      typename W::V t = x;
+      for (int k = 0; k <= 7; ++k) {
+        if (k == 1) t = y;
+        else if (k > 1) t = W::step(t, d);
+        acc.add(W::mul(x, y));
+        acc.sub(t);                // a single factor: the largest value a term can take
+        acc.sub(W::mul(t, z));
+        acc.add(t);
+      }
+    }
+    const Elem<L> r = acc.value();
+    for (int k = 0; k < L; ++k) out[i * L + k] = r.v[k];
+  }
+}
+
+extern "C" void field_host_work(int bit_width, int op, uint64_t n, const uint32_t* a, const uint32_t* b,
+                                const uint32_t* c, uint32_t* out) {
+  switch (bit_width) {
+    case 32: run_work<FieldP32>(op, n, a, b, c, out); break;
+    case 64: run_work<FieldP64>(op, n, a, b, c, out); break;
+    case 128: run_work<FieldP128>(op, n, a, b, c, out); break;
+    default: run_work<FieldP256>(op, n, a, b, c, out); break;
+  }
+}
+
+// Radix-2^29 helpers of the 255-bit field on raw limbs: op 0 unpack<0> then pack<0> (identity
+// for values < 2^256); op 1 top-limb reduction of the nine limbs in a (limbs < 2^30, top < 2^32).
+extern "C" void r29_host(int op, uint64_t n, const uint32_t* a, uint32_t* out) {
+  for (uint64_t i = 0; i < n; ++i) {
+    if (op == 0) {
+      r29_pack<0>(out + i * 8, r29_unpack<0>(a + i * 8));
+    } else {
+      R29 x;
+      for (int k = 0; k < 9; ++k) x.l[k] = a[i * 9 + k];
+      const R29 r = r29_reduce(x);
+      for (int k = 0; k < 9; ++k) out[i * 9 + k] = r.l[k];
+    }
   }
 }

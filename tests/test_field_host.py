@@ -28,6 +28,10 @@ def harness(tmp_path_factory):
     u32p = ctypes.POINTER(ctypes.c_uint32)
     lib.field_host_op.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, u32p, u32p, u32p]
     lib.field_host_op.restype = None
+    lib.field_host_work.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, u32p, u32p, u32p, u32p]
+    lib.field_host_work.restype = None
+    lib.r29_host.argtypes = [ctypes.c_int, ctypes.c_uint64, u32p, u32p]
+    lib.r29_host.restype = None
 
     def op(bw, code, a, b):
         f = get_field(bw)
@@ -36,6 +40,22 @@ def harness(tmp_path_factory):
         lib.field_host_op(bw, code, len(a), A.ctypes.data_as(u32p), B.ctypes.data_as(u32p), out.ctypes.data_as(u32p))
         return f.limbs_to_ints(out)
 
+    def work(bw, code, a, b, c):
+        f = get_field(bw)
+        A, B, C = f.ints_to_limbs(a), f.ints_to_limbs(b), f.ints_to_limbs(c)
+        out = np.zeros_like(A)
+        lib.field_host_work(bw, code, len(a), A.ctypes.data_as(u32p), B.ctypes.data_as(u32p),
+                            C.ctypes.data_as(u32p), out.ctypes.data_as(u32p))
+        return f.limbs_to_ints(out)
+
+    def r29(code, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.uint32)
+        out = np.zeros_like(arr)
+        lib.r29_host(code, arr.shape[0], arr.ctypes.data_as(u32p), out.ctypes.data_as(u32p))
+        return out
+
+    op.work = work
+    op.r29 = r29
     return op
 
 
@@ -66,3 +86,46 @@ def test_host_field_ops(harness, bw):
     assert harness(bw, 4, a, b) == want_mul  # product scanning (host form of the device MAC)
     assert harness(bw, 1, a, b) == [(x + y) % p for x, y in zip(a, b)]
     assert harness(bw, 2, a, b) == [(x - y) % p for x, y in zip(a, b)]
+
+
+@pytest.mark.parametrize("bw", [32, 64, 128, 256])
+def test_host_working_form(harness, bw):
+    """The kernels' working-form path (Montgomery in, working arithmetic, Montgomery out)
+    against Python integers, on edge and random operands; for the 255-bit field this is the
+    radix-2^29 representation with lazily reduced values."""
+    f = get_field(bw)
+    p, rinv = f.modulus, f.r_inv
+    a, b = operands(bw, n=1500)
+    rng = random.Random(bw + 11)
+    c = [rng.choice((0, 1, p - 1, rng.randrange(p))) for _ in a]
+    mont = lambda x, y: x * y * rinv % p
+    assert harness.work(bw, 0, a, b, c) == [mont(x, y) for x, y in zip(a, b)]
+    assert harness.work(bw, 1, a, b, c) == [(x + mont(z, y - x)) % p for x, y, z in zip(a, b, c)]
+    x7 = [(x + 7 * (y - x)) % p for x, y in zip(a, b)]
+    assert harness.work(bw, 2, a, b, c) == [(-mont(mont(mont(t, t), t), z)) % p for t, z in zip(x7, c)]
+    assert harness.work(bw, 3, a, b, c) == [39 * mont(x, y) % p for x, y in zip(a, b)]
+    want4 = []
+    for x, y, z in zip(a, b, c):
+        xs = sum((x + t * (y - x)) for t in range(8))
+        want4.append((8 * mont(x, y) - mont(z, xs % p)) % p)
+    assert harness.work(bw, 4, a, b, c) == want4
+
+
+def test_host_r29_pack_and_reduce(harness):
+    """Radix-2^29 helpers: unpack/pack round trip of 256-bit words, and the top-limb reduction
+    (result congruent mod p, below p + 2^241, limbs normalized) on limbs up to 2^30 with a top
+    limb up to 2^32."""
+    p = get_field(256).modulus
+    rng = np.random.default_rng(5)
+    w = rng.integers(0, 1 << 32, size=(4000, 8), dtype=np.uint64).astype(np.uint32)
+    w[:8] = 0xFFFFFFFF
+    w[8:16] = 0
+    assert np.array_equal(harness.r29(0, w), w)
+    limbs = rng.integers(0, 1 << 30, size=(4000, 9), dtype=np.uint64)
+    limbs[:, 8] = rng.integers(0, 1 << 32, size=4000, dtype=np.uint64)
+    limbs[:4] = [(1 << 30) - 1] * 8 + [(1 << 32) - 1]
+    out = harness.r29(1, limbs.astype(np.uint32))
+    for row_in, row_out in zip(limbs.tolist(), out.tolist()):
+        v = sum(int(x) << (29 * i) for i, x in enumerate(row_in))
+        r = sum(int(x) << (29 * i) for i, x in enumerate(row_out))
+        assert r % p == v % p and r < p + (1 << 241) and all(x < (1 << 29) for x in row_out)

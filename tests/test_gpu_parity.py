@@ -64,6 +64,9 @@ def test_field_ops_match_big_integers(ctx, bw):
     assert f.limbs_to_ints(ctx.field_op(bw, "sub", A, B)) == [(x - y) % p for x, y in zip(a, b)]
     assert f.limbs_to_ints(ctx.field_op(bw, "to_mont", A)) == [x * f.r_mod % p for x in a]
     assert f.limbs_to_ints(ctx.field_op(bw, "from_mont", A)) == [x * rinv % p for x in a]
+    # the prover kernels' internal arithmetic (radix 2^29 for the 255-bit field)
+    assert f.limbs_to_ints(ctx.field_op(bw, "work_mul", A, B)) == [x * y * rinv % p for x, y in zip(a, b)]
+    assert f.limbs_to_ints(ctx.field_op(bw, "work_fold", A, B)) == [(x + x * (y - x) * rinv) % p for x, y in zip(a, b)]
 
 
 def test_bls_montgomery_product_like_reference_test(ctx):
