@@ -62,6 +62,46 @@ SC_HD void keccak_f1600(uint64_t A[25]) {
   }
 }
 
+#if defined(__HIPCC__)
+// Keccak-f[1600] spread over one wave: lane x + 5y (< 25) holds A[x + 5y] as two 32-bit
+// halves, and the column sums of theta, the pi permutation and the row terms of chi are lane
+// exchanges. Lanes 25..63 follow along on copies of lane 0..24 data and are ignored.
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+  const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, 64);
+  const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, 64);
+  return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
+__device__ __forceinline__ uint64_t rotl64_var(uint64_t x, int s) {
+  return s == 0 ? x : ((x << s) | (x >> (64 - s)));
+}
+
+__device__ __forceinline__ void keccak_f1600_wave(uint64_t& a) {
+  const int lane = (int)(threadIdx.x & 63) % 25;
+  const int x = lane % 5, y = lane / 5;
+  // rho offset of this lane's word, and the lane whose word pi moves here: B[y, 2x+3y] =
+  // rot(A[x, y]) means lane (X, Y) receives from (x, y) = ((Y - 3X) * 3 mod 5, X) after rho.
+  constexpr int R[25] = {0,  1,  62, 28, 27, 36, 44, 6,  55, 20, 3,  10, 43,
+                         25, 39, 41, 45, 15, 21, 8,  18, 2,  61, 56, 14};
+  const int rot = R[lane];
+  const int pi_src = ((3 * ((y - 3 * x + 15) % 5)) % 5) + 5 * x;
+  const int c1 = (x + 4) % 5, c2 = (x + 1) % 5;
+  const int n1 = (x + 1) % 5 + 5 * y, n2 = (x + 2) % 5 + 5 * y;
+  for (int round = 0; round < 24; ++round) {
+    // theta: column parity C[x], then D[x] = C[x-1] ^ rot(C[x+1], 1)
+    uint64_t c = a;
+#pragma unroll
+    for (int k = 1; k < 5; ++k) c ^= shfl64(a, x + 5 * ((y + k) % 5));
+    a ^= shfl64(c, c1) ^ rotl64(shfl64(c, c2), 1);
+    // rho and pi
+    const uint64_t b = shfl64(rotl64_var(a, rot), pi_src);
+    // chi and iota
+    a = b ^ (~shfl64(b, n1) & shfl64(b, n2));
+    if (lane == 0 && (threadIdx.x & 63) == 0) a ^= keccak_rc(round);
+  }
+}
+#endif
+
 // SHA3-256 over a message already padded (pad10*1, SHA3 domain bits) into whole
 // 136-byte blocks of 34 little-endian 32-bit words.
 SC_HD void sha3_256_padded(const uint32_t* words, int nblocks, uint32_t digest[8]) {

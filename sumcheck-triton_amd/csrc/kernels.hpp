@@ -116,8 +116,8 @@ __device__ __forceinline__ void compose_add(typename Work<F>::Acc& acc, const ty
   for (int k = 0; k < K; ++k) {
     typename W::V prod = x[0];  // every term has a factor, so prod is set before it is used
     bool have = false;
-#pragma unroll
-    for (int v = 0; v < TMAX; ++v) {
+    table_for<TMAX, (TMAX <= 8)>([&](auto vi) {
+      const int v = vi;
       uint32_t e = (cd->exps[k][v >> 3] >> ((v & 7) * 4)) & 15u;
       for (; e != 0u; --e) {
         if (have) {
@@ -127,7 +127,7 @@ __device__ __forceinline__ void compose_add(typename Work<F>::Acc& acc, const ty
           have = true;
         }
       }
-    }
+    });
     const int kind = cd->kind[k];
     if (kind == COEFF_ONE) {
       acc.add(prod);
@@ -197,7 +197,7 @@ __device__ __forceinline__ void eval_points(typename Work<F>::V (&cur)[TMAX], ty
                                             typename Work<F>::Acc (&acc)[PMAX], const CompDesc* __restrict__ cd,
                                             int T, int K, int npts) {
   using W = Work<F>;
-  constexpr bool X = SID != 0;  // compiled compositions expand every loop
+  constexpr bool X = SID != 0 || TMAX <= 8;  // expand every loop unless the table count is large
   compose<F, SID, TMAX>(acc[0], cur, cd, K);
   table_for<TMAX, X>([&](auto vi) {
     const int v = vi;
@@ -298,24 +298,20 @@ __device__ __forceinline__ void block_reduce(Elem<F::L> (&acc)[PMAX], int npts, 
   __syncthreads();
 }
 
-// SHA3-256 of a padded message held in LDS, run by one whole wave on wave-uniform values.
+// SHA3-256 of a padded message held in LDS, run by one whole wave (lane k of the first 25
+// holds state word k); the digest is returned to every lane.
 __device__ __forceinline__ void sha3_uniform(const uint32_t* msg, int nblocks, uint32_t digest[8]) {
-  uint64_t A[25];
-#pragma unroll
-  for (int i = 0; i < 25; ++i) A[i] = 0;
+  const int lane = threadIdx.x & 63;
+  uint64_t a = 0;
   for (int b = 0; b < nblocks; ++b) {
-#pragma unroll
-    for (int i = 0; i < 17; ++i) {
-      const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)msg[34 * b + 2 * i]);
-      const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)msg[34 * b + 2 * i + 1]);
-      A[i] ^= (uint64_t)lo | ((uint64_t)hi << 32);
-    }
-    keccak_f1600(A);
+    if (lane < 17) a ^= (uint64_t)msg[34 * b + 2 * lane] | ((uint64_t)msg[34 * b + 2 * lane + 1] << 32);
+    keccak_f1600_wave(a);
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    digest[2 * i] = (uint32_t)A[i];
-    digest[2 * i + 1] = (uint32_t)(A[i] >> 32);
+    const uint64_t w = shfl64(a, i);
+    digest[2 * i] = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)w);
+    digest[2 * i + 1] = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(w >> 32));
   }
 }
 

@@ -14,9 +14,10 @@
 //   fold(lo, hi, r)   lo + r (hi - lo), the folded entry, ready for store
 //   Acc               running sum: add, sub, value() -> canonical Montgomery limbs
 //
-// The 2^32-5, Goldilocks and 2^127-1 fields work on their Montgomery limbs directly. The
-// BLS12-381 scalar field works in radix 2^29 (R29 below): a 32x32-bit multiply-add then never
-// overflows a 64-bit column sum, so Montgomery multiplication needs no carry chains.
+// The 2^32-5 and Goldilocks fields work on their Montgomery limbs directly. 2^127-1 works on
+// plain residues in radix 2^26 (R26) and the BLS12-381 scalar field on Montgomery images with
+// R = 2^261 in radix 2^29 (R29): with limbs narrower than 32 bits, 32x32-bit multiply-adds
+// never overflow a 64-bit column sum, so products need no carry chains.
 #pragma once
 
 #include "field.hpp"
@@ -301,6 +302,185 @@ struct Work<FieldP256> {
     SC_HD Elem<8> value() const { return r29_to_mont(r29_reduce(v)); }
   };
 };
+
+// ---------------------------------------------------------------------------------------------
+// R26: 2^127 - 1 in five 26-bit limbs holding the plain residue (not a Montgomery image).
+//
+// The Montgomery image of x is X = 2x mod p (R = 2^128 = 2), so conversion is a one-bit
+// rotation within 127 bits each way. Products fold at 2^130 = 8 (mod p) and every result
+// has its bits at and above 2^127 folded back (2^127 = 1), so values stay below 2^127 + 2^5
+// with limbs below 2^26 + 2^5: column sums of five products stay far below 2^64.
+// ---------------------------------------------------------------------------------------------
+struct R26 {
+  uint32_t l[5];
+};
+
+namespace r26 {
+constexpr uint32_t MASK = (1u << 26) - 1;
+constexpr uint32_t PAD2 = 0x7FFFFFEu;   // 2p with limbs 0..3 in [2^26, 2^27)
+constexpr uint32_t PAD2_TOP = 0xFFFFFEu;
+}  // namespace r26
+
+// Carries of limbs 0..3 moved up, then bits 127 and above of the top limb folded into limb 0.
+SC_HD R26 r26_normalize(R26 x) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    x.l[i + 1] += x.l[i] >> 26;
+    x.l[i] &= r26::MASK;
+  }
+  x.l[0] += x.l[4] >> 23;
+  x.l[4] &= (1u << 23) - 1;
+  return x;
+}
+
+// Four words of a value below 2^128 -> limbs.
+SC_HD R26 r26_unpack(const uint32_t* w) {
+  R26 r;
+  r.l[0] = w[0] & r26::MASK;
+  r.l[1] = ((w[0] >> 26) | (w[1] << 6)) & r26::MASK;
+  r.l[2] = ((w[1] >> 20) | (w[2] << 12)) & r26::MASK;
+  r.l[3] = ((w[2] >> 14) | (w[3] << 18)) & r26::MASK;
+  r.l[4] = w[3] >> 8;
+  return r;
+}
+
+// Limbs (each below 2^27, value below 2^128) -> four words, carries resolved on the way.
+SC_HD void r26_pack(uint32_t* w, const R26& x) {
+  uint64_t acc = x.l[0] + ((uint64_t)x.l[1] << 26);
+  w[0] = (uint32_t)acc;
+  acc = (acc >> 32) + ((uint64_t)x.l[2] << 20);
+  w[1] = (uint32_t)acc;
+  acc = (acc >> 32) + ((uint64_t)x.l[3] << 14);
+  w[2] = (uint32_t)acc;
+  acc = (acc >> 32) + ((uint64_t)x.l[4] << 8);
+  w[3] = (uint32_t)acc;
+}
+
+// a * b mod p (lazily reduced): ten columns, the upper five folded down with 2^130 = 8.
+SC_HD R26 r26_mul(const R26& a, const R26& b) {
+  uint64_t c[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = (k < 5 ? 0 : k - 4); i <= (k < 5 ? k : 4); ++i) s += (uint64_t)a.l[i] * b.l[k - i];
+    c[k] = s;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) c[k] += c[k + 5] << 3;
+  R26 r;
+  uint64_t carry = 0;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const uint64_t s = c[k] + carry;
+    r.l[k] = (uint32_t)s & r26::MASK;
+    carry = s >> 26;
+  }
+  // carry has weight 2^130 = 8; it is below 2^35, so 8 * carry fits after one more step
+  const uint64_t s0 = (uint64_t)r.l[0] + (carry << 3);
+  r.l[0] = (uint32_t)s0 & r26::MASK;
+  r.l[1] += (uint32_t)(s0 >> 26);
+  r.l[0] += r.l[4] >> 23;
+  r.l[4] &= (1u << 23) - 1;
+  return r;
+}
+
+// Plain residue (value below 2^127 + 2^5) -> Montgomery limbs 2x mod p in [0, p).
+SC_HD Elem<4> r26_to_mont(const R26& x) {
+  uint32_t w[4];
+  r26_pack(w, r26_normalize(x));
+  // value v <= 2^127 + small: v >= p  <=>  v + 1 >= 2^127
+  uint64_t c = (uint64_t)w[0] + 1u;
+  uint32_t y[4];
+  y[0] = (uint32_t)c;
+#pragma unroll
+  for (int i = 1; i < 4; ++i) {
+    c = (c >> 32) + w[i];
+    y[i] = (uint32_t)c;
+  }
+  if (y[3] >> 31) {  // v - p = v + 1 - 2^127
+    w[0] = y[0];
+    w[1] = y[1];
+    w[2] = y[2];
+    w[3] = y[3] & 0x7FFFFFFFu;
+  }
+  Elem<4> out;  // rotate left by one within 127 bits
+  out.v[0] = (w[0] << 1) | (w[3] >> 30);
+  out.v[1] = (w[1] << 1) | (w[0] >> 31);
+  out.v[2] = (w[2] << 1) | (w[1] >> 31);
+  out.v[3] = ((w[3] << 1) | (w[2] >> 31)) & 0x7FFFFFFFu;
+  return out;
+}
+
+template <>
+struct Work<FieldP128> {
+  static constexpr int L = 4;
+  using V = R26;
+  // Montgomery X = 2x (X < p): x is X rotated right by one within 127 bits.
+  SC_HD static V from_mont(const uint32_t* w) {
+    uint32_t x[4];
+    x[0] = (w[0] >> 1) | (w[1] << 31);
+    x[1] = (w[1] >> 1) | (w[2] << 31);
+    x[2] = (w[2] >> 1) | (w[3] << 31);
+    x[3] = (w[3] >> 1) | ((w[0] & 1u) << 30);
+    return r26_unpack(x);
+  }
+  SC_HD static V from_stored(const uint32_t* w) { return r26_unpack(w); }
+  SC_HD static V coeff(const uint32_t* w) { return r26_unpack(w); }
+  SC_HD static void store(uint32_t* w, const V& x) { r26_pack(w, x); }
+  SC_HD static V mul(const V& a, const V& b) { return r26_mul(a, b); }
+  SC_HD static V diff(const V& hi, const V& lo) {  // hi + 2p - lo
+    R26 d;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) d.l[i] = hi.l[i] + r26::PAD2 - lo.l[i];
+    d.l[4] = hi.l[4] + r26::PAD2_TOP - lo.l[4];
+    return r26_normalize(d);
+  }
+  SC_HD static V step(const V& x, const V& d) {
+    R26 s;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) s.l[i] = x.l[i] + d.l[i];
+    return r26_normalize(s);
+  }
+  SC_HD static V fold(const V& lo, const V& hi, const V& r) {
+    const R26 m = r26_mul(r, diff(hi, lo));
+    R26 s;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) s.l[i] = lo.l[i] + m.l[i];
+    return r26_normalize(s);
+  }
+  struct Acc {
+    R26 v;
+    SC_HD void clear() {
+#pragma unroll
+      for (int i = 0; i < 5; ++i) v.l[i] = 0u;
+    }
+    SC_HD void add(const R26& x) {
+#pragma unroll
+      for (int i = 0; i < 5; ++i) v.l[i] += x.l[i];
+      v = r26_normalize(v);
+    }
+    SC_HD void sub(const R26& x) {
+      R26 s;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) s.l[i] = v.l[i] + r26::PAD2 - x.l[i];
+      s.l[4] = v.l[4] + r26::PAD2_TOP - x.l[4];
+      v = r26_normalize(s);
+    }
+    SC_HD Elem<4> value() const { return r26_to_mont(v); }
+  };
+};
+
+template <>
+inline Elem<4> work_coeff<FieldP128>(const Elem<4>& c) {
+  // the plain residue c / 2 mod p, as stored words
+  Elem<4> x;
+  x.v[0] = (c.v[0] >> 1) | (c.v[1] << 31);
+  x.v[1] = (c.v[1] >> 1) | (c.v[2] << 31);
+  x.v[2] = (c.v[2] >> 1) | (c.v[3] << 31);
+  x.v[3] = (c.v[3] >> 1) | ((c.v[0] & 1u) << 30);
+  return x;
+}
 
 template <>
 inline Elem<8> work_coeff<FieldP256>(const Elem<8>& c) {
