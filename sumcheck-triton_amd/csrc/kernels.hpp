@@ -491,6 +491,93 @@ __global__ void __launch_bounds__(TB) tail_kernel(TailArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// Wide compositions (more than 8 tables, interpreted): the per-pair kernels above would hold
+// 2 x tables working values per thread and spill. These rounds run as two launches instead:
+// wide_fold_kernel writes round j's tables (fold of round j-1 with r_{j-1}); wide_eval_kernel
+// gives each pair G = PMAX lanes, lane t evaluating the composition at point t only, so a
+// lane holds one value per table and one accumulator. Same partials layout as pass_kernel.
+// ---------------------------------------------------------------------------------------------
+template <class F>
+__global__ void __launch_bounds__(BLOCK) wide_fold_kernel(PassArgs a) {
+  using W = Work<F>;
+  constexpr int L = F::L;
+  const int T = a.comp->num_tables;
+  const uint64_t n_new = a.src_n >> 1;  // a power of two
+  const int shift = __builtin_ctzll(n_new);
+  const uint64_t total = n_new * (uint64_t)T;
+  const bool src_mont = a.round <= 1;
+  const typename W::V r = W::from_mont(a.st->r[a.round - 1]);
+  for (uint64_t idx = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; idx < total;
+       idx += (uint64_t)gridDim.x * BLOCK) {
+    const uint64_t v = idx >> shift, i = idx & (n_new - 1);
+    uint32_t w[2 * L];
+    load_words<2 * L>(w, a.src + (v * a.src_n + 2 * i) * L);
+    const typename W::V lo = src_mont ? W::from_mont(w) : W::from_stored(w);
+    const typename W::V hi = src_mont ? W::from_mont(w + L) : W::from_stored(w + L);
+    uint32_t y[L];
+    W::store(y, W::fold(lo, hi, r));
+    uint32_t* d = a.dst + (v * n_new + i) * L;
+#pragma unroll
+    for (int k = 0; k < L; ++k) d[k] = y[k];
+  }
+}
+
+template <class F, int TMAX, int G>
+__global__ void __launch_bounds__(BLOCK) wide_eval_kernel(PassArgs a) {
+  using W = Work<F>;
+  constexpr int L = F::L;
+  constexpr int PAIRS = BLOCK / G;  // pairs per block per sweep
+  __shared__ uint32_t s_red[(BLOCK / 64) * G * L];
+  const CompDesc* __restrict__ cd = a.comp;
+  const int T = cd->num_tables;
+  const int K = cd->num_terms;
+  const int npts = cd->npts;
+  const int t = threadIdx.x % G;
+  const bool src_mont = a.round == 0;
+  typename W::Acc acc;
+  acc.clear();
+  const uint64_t stride = (uint64_t)gridDim.x * PAIRS;
+  for (uint64_t i = (uint64_t)blockIdx.x * PAIRS + threadIdx.x / G; i < a.pairs; i += stride) {
+    typename W::V x[TMAX];
+    static_for<TMAX>([&](auto vi) {
+      constexpr int v = decltype(vi)::value;
+      if (v < T) {
+        uint32_t w[2 * L];
+        load_words<2 * L>(w, a.src + ((uint64_t)v * a.src_n + 2 * i) * L);
+        const typename W::V lo = src_mont ? W::from_mont(w) : W::from_stored(w);
+        const typename W::V hi = src_mont ? W::from_mont(w + L) : W::from_stored(w + L);
+        x[v] = W::at(lo, W::diff(hi, lo), t);
+      } else {
+        x[v] = x[0];
+      }
+    });
+    if (t < npts) compose_add<F, TMAX>(acc, x, cd, K);
+  }
+  // Lanes whose index differs by a multiple of G hold the same point.
+  Elem<L> sum = acc.value();
+#pragma unroll
+  for (int off = G; off < 64; off <<= 1) {
+    Elem<L> o;
+#pragma unroll
+    for (int k = 0; k < L; ++k) o.v[k] = __shfl_xor(sum.v[k], off, 64);
+    sum = add<F>(sum, o);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane < G) {
+#pragma unroll
+    for (int k = 0; k < L; ++k) s_red[(wave * G + lane) * L + k] = sum.v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < npts) {
+    Elem<L> s2 = load_elem<L>(s_red + threadIdx.x * L);
+#pragma unroll
+    for (int w = 1; w < BLOCK / 64; ++w) s2 = add<F>(s2, load_elem<L>(s_red + (w * G + threadIdx.x) * L));
+#pragma unroll
+    for (int k = 0; k < L; ++k) a.partials[((uint64_t)blockIdx.x * MAX_POINTS + threadIdx.x) * L + k] = s2.v[k];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Batched stages
 // ---------------------------------------------------------------------------------------------
 

@@ -22,6 +22,10 @@ struct Launchers {
   // batched stages: grid = tiles of the stage / entries of the stage
   hipError_t (*batch_pass)(hipStream_t, int tmax, int pmax, const BatchArgs&, unsigned tiles);
   hipError_t (*batch_finalize)(hipStream_t, int pmax, const BatchArgs&, unsigned entries);
+  // wide interpreted compositions (tmax > 8): fold launch (round >= 1), then a point-parallel
+  // evaluation launch with pmax lanes per pair
+  hipError_t (*wide_fold)(hipStream_t, const PassArgs&, unsigned grid);
+  hipError_t (*wide_eval)(hipStream_t, int tmax, int pmax, const PassArgs&, unsigned grid);
 };
 
 const Launchers& launchers_p32();

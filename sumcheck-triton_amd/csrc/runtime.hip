@@ -283,10 +283,28 @@ int enqueue_rounds(sc_proof* p, int mode) {
     a.pad0 = 0;
     uint64_t g = (a.pairs + BLOCK - 1) / BLOCK;
     if (g > (uint64_t)p->max_pass_blocks) g = (uint64_t)p->max_pass_blocks;
+    // Interpreted compositions of more than 8 tables: fold launch + point-parallel evaluation.
+    const bool wide = p->sid == 0 && p->tmax > 8;
     if (p->profiling) SC_HIP(hipEventRecord(p->prof_ev[2 * j], s));
-    SC_HIP(p->ln->pass(s, p->sid, p->tmax, p->pmax, a, (unsigned)g));
+    if (wide) {
+      if (j > 0) {
+        uint64_t gf = ((a.src_n >> 1) * (uint64_t)p->T + BLOCK - 1) / BLOCK;
+        SC_HIP(p->ln->wide_fold(s, a, (unsigned)std::min<uint64_t>(gf, 8192)));
+      }
+      PassArgs e = a;
+      e.src = buf_for_round(p, j);
+      e.dst = nullptr;
+      e.src_n = p->n0 >> j;
+      e.fold = 0;
+      e.fused_finalize = 0;
+      const uint64_t pairs_per_block = BLOCK / p->pmax;
+      g = std::min<uint64_t>((a.pairs + pairs_per_block - 1) / pairs_per_block, (uint64_t)p->max_pass_blocks);
+      SC_HIP(p->ln->wide_eval(s, p->tmax, p->pmax, e, (unsigned)g));
+    } else {
+      SC_HIP(p->ln->pass(s, p->sid, p->tmax, p->pmax, a, (unsigned)g));
+    }
     if (p->profiling) SC_HIP(hipEventRecord(p->prof_ev[2 * j + 1], s));
-    if (p->fused_finalize) continue;
+    if (p->fused_finalize && !wide) continue;
     FinalizeArgs f;
     f.partials = p->d_partials;
     f.nblocks = (uint32_t)g;

@@ -12,6 +12,7 @@
 //   diff(hi, lo)      hi - lo, the interpolation step of a pair
 //   step(x, d)        x + d
 //   fold(lo, hi, r)   lo + r (hi - lo), the folded entry, ready for store
+//   at(lo, d, t)      lo + t d for a small per-lane t (point-parallel evaluation)
 //   Acc               running sum: add, sub, value() -> canonical Montgomery limbs
 //
 // The 2^32-5 and Goldilocks fields work on their Montgomery limbs directly. 2^127-1 works on
@@ -45,6 +46,12 @@ struct Work {
   SC_HD static V diff(const V& hi, const V& lo) { return sub<F>(hi, lo); }
   SC_HD static V step(const V& x, const V& d) { return step_add<F>(x, d); }
   SC_HD static V fold(const V& lo, const V& hi, const V& r) { return add<F>(lo, mont_mul<F>(r, sub<F>(hi, lo))); }
+  // lo + t d for a per-lane 0 <= t < 8
+  SC_HD static V at(const V& lo, const V& d, int t) {
+    V x = lo;
+    for (int s = 0; s < t; ++s) x = step_add<F>(x, d);
+    return x;
+  }
   using Acc = Accum<F>;
 };
 
@@ -270,6 +277,13 @@ struct Work<FieldP256> {
     for (int i = 0; i < 9; ++i) s.l[i] = x.l[i] + d.l[i];
     return r29_normalize(s);
   }
+  // lo + t d for 0 <= t < 8 (limbs stay below 2^32; the value below 29p as for t steps)
+  SC_HD static V at(const V& lo, const V& d, int t) {
+    R29 x;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) x.l[i] = lo.l[i] + (uint32_t)t * d.l[i];
+    return r29_normalize(x);
+  }
   // lo + r (hi + 3p - lo), reduced below p + 2^241. The difference is passed to the product
   // with limbs < 2^31 (r's are < 2^29), and the sum's carries are resolved by the reduction.
   SC_HD static V fold(const V& lo, const V& hi, const V& r) {
@@ -441,6 +455,12 @@ struct Work<FieldP128> {
 #pragma unroll
     for (int i = 0; i < 5; ++i) s.l[i] = x.l[i] + d.l[i];
     return r26_normalize(s);
+  }
+  SC_HD static V at(const V& lo, const V& d, int t) {  // 0 <= t < 8
+    R26 x;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) x.l[i] = lo.l[i] + (uint32_t)t * d.l[i];
+    return r26_normalize(x);
   }
   SC_HD static V fold(const V& lo, const V& hi, const V& r) {
     const R26 m = r26_mul(r, diff(hi, lo));

@@ -178,6 +178,35 @@ def test_random_field_coefficients_and_selectors(ctx, bw):
 
 
 @pytest.mark.parametrize("bw", WIDTHS)
+def test_wide_compositions_match_c_oracle(ctx, bw):
+    """Interpreted compositions of more than 8 tables run as a fold launch plus a
+    point-parallel evaluation launch per round: 12 tables (degree 6, 8 lanes per pair) and
+    20 tables (degree 3, 4 lanes per pair), under the default schedule and with every round
+    outside the single-workgroup tail, against the C oracle."""
+    from oracle import cbind
+
+    f = get_field(bw)
+    rng = random.Random(900 + bw)
+    for T, degs, nterms in ((12, (3, 7), 12), (20, (2, 4), 9)):
+        terms = [(rng.randrange(f.modulus), [rng.randrange(T) for _ in range(rng.randrange(*degs))])
+                 for _ in range(nterms)]
+        terms[0] = (1, terms[0][1])
+        terms[1] = (f.modulus - 1, terms[1][1])
+        comp = Composition.from_terms(T, terms)
+        nv = 12
+        mask = (1 << (T - 1)) | (1 << (T - 2))
+        want_g, want_r, _ = cbind.prove(bw, T, _oracle_terms(comp), nv, seed=41, selector_mask=mask)
+        with Proof(ctx, bw, comp, nv, input="uniform", seed=41, selector_mask=mask) as pr:
+            for blocks, tail in ((0, 0), (16, 1), (4096, 1)):
+                say("wide", bw, T, blocks, tail)
+                pr.set_schedule(blocks, tail, False)
+                pr.run("sha3")
+                pr.wait()
+                g, r, _cs = pr.result()
+                assert np.array_equal(g, want_g) and np.array_equal(r, want_r), (T, blocks, tail)
+
+
+@pytest.mark.parametrize("bw", WIDTHS)
 def test_package_api_matches_oracle_reference_contract(bw):
     """tests/test_zkduel.py:54-82 of the reference, at every width: explicit challenges."""
     f = get_field(bw)
