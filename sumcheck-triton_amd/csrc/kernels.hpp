@@ -109,14 +109,14 @@ __device__ __forceinline__ Elem<L> load_elem(const uint32_t* p) {
 
 // C(x) = sum_k c_k prod_v x_v^e_kv. Term structure is wave-uniform, so every branch below is
 // a scalar branch; table slots are indexed statically (unrolled v), keeping x in VGPRs.
-template <class F, int TMAX>
+template <class F, int TMAX, bool EXPAND = (TMAX <= 8)>
 __device__ __forceinline__ void compose_add(typename Work<F>::Acc& acc, const typename Work<F>::V (&x)[TMAX],
                                             const CompDesc* __restrict__ cd, int K) {
   using W = Work<F>;
   for (int k = 0; k < K; ++k) {
     typename W::V prod = x[0];  // every term has a factor, so prod is set before it is used
     bool have = false;
-    table_for<TMAX, (TMAX <= 8)>([&](auto vi) {
+    table_for<TMAX, EXPAND>([&](auto vi) {
       const int v = vi;
       uint32_t e = (cd->exps[k][v >> 3] >> ((v & 7) * 4)) & 15u;
       for (; e != 0u; --e) {
@@ -551,6 +551,8 @@ __global__ void __launch_bounds__(BLOCK) wide_eval_kernel(PassArgs a) {
         x[v] = x[0];
       }
     });
+    // The term loop over TMAX slots stays a loop here (x in scratch, the rest in registers):
+    // expanding it needs 1 wave per SIMD and measured slower.
     if (t < npts) compose_add<F, TMAX>(acc, x, cd, K);
   }
   // Lanes whose index differs by a multiple of G hold the same point.

@@ -244,7 +244,11 @@ int default_pass_blocks(int L) {
   }
 }
 
-int64_t default_tail_pairs(int L) {
+// Pairs below which the remaining rounds run in the single-workgroup tail kernel. Compositions
+// of more than 8 tables skip it: their per-pair work is large enough that one workgroup is
+// slower than the fold + point-parallel launches even for a handful of pairs.
+int64_t default_tail_pairs(int L, int tmax) {
+  if (tmax > 8) return 1;
   return L == 8 ? 64 : 512;
 }
 
@@ -465,7 +469,7 @@ int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* co
   p->tmax = pick_tmax(p->T);
   p->pmax = p->npts <= 4 ? 4 : 8;
   p->n0 = (uint64_t)1 << num_vars;
-  p->tail_pairs = default_tail_pairs(p->L);
+  p->tail_pairs = default_tail_pairs(p->L, p->tmax);
   p->max_pass_blocks = default_pass_blocks(p->L);
   p->matched_sid = match_static(desc);
   p->sid = p->matched_sid;
@@ -542,7 +546,7 @@ int sc_proof_set_schedule(sc_proof* p, int32_t max_pass_blocks, int64_t tail_pai
     return fail(SC_ERR_ARG, "max_pass_blocks must be in [0, 4096]");
   if (tail_pairs < 0) return fail(SC_ERR_ARG, "tail_pairs must be >= 0");
   p->max_pass_blocks = max_pass_blocks ? max_pass_blocks : default_pass_blocks(p->L);
-  p->tail_pairs = tail_pairs ? tail_pairs : default_tail_pairs(p->L);
+  p->tail_pairs = tail_pairs ? tail_pairs : default_tail_pairs(p->L, p->tmax);
   if (p->graph_exec) {
     (void)hipGraphExecDestroy(p->graph_exec);
     (void)hipGraphDestroy(p->graph);
