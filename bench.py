@@ -143,15 +143,8 @@ def measure_batched(ctx, bw: int, n: int = 256, reps: int = 3) -> dict:
 
     insts = batch_instances(n, seed=4)
     with Batch(ctx, bw, insts) as b:
-        tune = {}
-        for items in (1, 2, 4, 8):
-            b.set_tile(items)
-            b.run("sha3")
-            b.wait()
-            b.run("sha3")
-            tune[items] = b.wait()
+        tune = b.autotune((1, 2, 4, 8), "sha3")
         best = min(tune, key=tune.get)
-        b.set_tile(best)
         b.capture("sha3")
         b.run("sha3")
         b.wait()

@@ -325,6 +325,24 @@ class Batch:
                                             nvs.ctypes.data_as(_lib.i32p), inps, ctypes.byref(handle)))
         self.handle = handle
 
+    def autotune(self, candidates: Sequence[int] = (1, 2, 4, 8), challenge_mode: str = "sha3",
+                 repeats: int = 1) -> dict[int, float]:
+        """Times one batch run per tile size (pairs per thread) after a warm-up run and keeps
+        the fastest; returns {items: ms}. The tile changes only the schedule, never the
+        results."""
+        times: dict[int, float] = {}
+        for items in candidates:
+            self.set_tile(items)
+            self.run(challenge_mode)
+            self.wait()
+            best = float("inf")
+            for _ in range(max(1, repeats)):
+                self.run(challenge_mode)
+                best = min(best, self.wait())
+            times[int(items)] = best
+        self.set_tile(min(times, key=times.get))
+        return times
+
     def close(self) -> None:
         if self.handle:
             self.lib.sc_batch_destroy(self.handle)
