@@ -124,6 +124,13 @@ int sc_proof_use_compiled(sc_proof* proof, int32_t enable);
 int sc_proof_set_profiling(sc_proof* proof, int32_t enable);
 int sc_proof_kernel_times(sc_proof* proof, float* pass_ms, int32_t capacity, float* tail_ms);
 
+/* Final-point step of the last run (survey 8(f) item 4; the reference stops after the last
+ * round polynomial, field_sumcheck.cu:1057): r_final [L] = the challenge derived from the last
+ * round's polynomial by the same transcript rule, final_values [T][L] = every table folded at
+ * (r_0, ..., r_final), i.e. f_v at the random point. Montgomery limbs. A verifier checks
+ * g_last(r_final) = C(final_values). SC_ERR_UNSUPPORTED after a run with explicit challenges. */
+int sc_proof_final(sc_proof* proof, uint32_t* r_final, uint32_t* final_values);
+
 int sc_proof_destroy(sc_proof* proof);
 
 /* One-shot: create, run, wait, read results, destroy. */

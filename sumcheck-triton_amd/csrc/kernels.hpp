@@ -316,15 +316,17 @@ __device__ __forceinline__ void sha3_uniform(const uint32_t* msg, int nblocks, u
 }
 
 // Publishes g (LDS, npts*L words) as round `round`'s polynomial and, unless it is the last
-// round, derives r_round into r_out (LDS, L words) and st->r[round]. All threads call it.
+// round (or last_challenge asks for the final-point challenge too), derives r_round into
+// r_out (LDS, L words) and st->r[round]. All threads call it.
 template <class F, int NT>
 __device__ __forceinline__ void finish_round(const uint32_t* g, int npts, int round, int num_rounds, int mode,
                              const FieldConsts* __restrict__ fc, ProofState* st,
-                             const uint32_t* __restrict__ explicit_r, uint32_t* msg, uint32_t* r_out) {
+                             const uint32_t* __restrict__ explicit_r, uint32_t* msg, uint32_t* r_out,
+                             bool last_challenge = false) {
   constexpr int L = F::L;
   const int tid = threadIdx.x;
   if (tid < npts * L) st->polys[round][tid / L][tid % L] = g[tid];
-  if (round + 1 >= num_rounds) return;
+  if (round + 1 >= num_rounds && !last_challenge) return;
   if (mode == CHALLENGE_SHA3) {
     // Digest of the transcript, read as a little-endian integer, reduced mod p and
     // Montgomery-encoded (field_sumcheck.cu:857-907): sum_i digest_i * 2^(32i) * R mod p.
@@ -485,9 +487,35 @@ __global__ void __launch_bounds__(TB) tail_kernel(TailArgs a) {
 #pragma unroll
     for (int t = 0; t < PMAX; ++t) sums[t] = acc[t].value();
     block_reduce<F, PMAX, TB>(sums, npts, s_red, s_g);
-    finish_round<F, TB>(s_g, npts, j, a.num_rounds, a.challenge_mode, a.fc, a.st, a.explicit_r, s_msg, s_r);
+    finish_round<F, TB>(s_g, npts, j, a.num_rounds, a.challenge_mode, a.fc, a.st, a.explicit_r, s_msg, s_r,
+                        a.want_final != 0);
     __syncthreads();
   }
+  if (!a.want_final) return;
+  // Final-point step: fold the last round's pair of every table with the last challenge.
+  // The last round's entries are in its output buffer, or the input when there is one round.
+  const int jl = a.num_rounds - 1;
+  if (threadIdx.x < T) {
+    using W = Work<F>;
+    const int v = threadIdx.x;
+    typename W::V lo, hi;
+    if (jl == 0) {
+      lo = W::from_mont(a.input + (uint64_t)v * 2 * L);
+      hi = W::from_mont(a.input + ((uint64_t)v * 2 + 1) * L);
+    } else {
+      const uint32_t* last = (jl & 1) ? a.buf_a : a.buf_b;
+      lo = W::from_stored(last + (uint64_t)v * 2 * L);
+      hi = W::from_stored(last + ((uint64_t)v * 2 + 1) * L);
+    }
+    typename W::Acc f;
+    f.clear();
+    f.add(W::fold(lo, hi, W::from_mont(s_r)));
+    const Elem<L> m = f.value();
+#pragma unroll
+    for (int k = 0; k < L; ++k) a.st->final_vals[v][k] = m.v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) a.st->final_ready = 1u;
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -73,6 +73,11 @@ struct ProofState {
   uint32_t pad[63];
   uint32_t r[MAX_ROUNDS][MAX_LIMBS];                     // challenge after round j, Montgomery
   uint32_t polys[MAX_ROUNDS][MAX_POINTS][MAX_LIMBS];     // g_j(t), Montgomery
+  // Final-point step (after the last round): r[num_rounds-1] is the last challenge and
+  // final_vals[v] = f_v(r_0, ..., r_{num_rounds-1}), Montgomery; final_ready = 1 once written.
+  uint32_t final_ready;
+  uint32_t pad2[63];
+  uint32_t final_vals[MAX_TABLES][MAX_LIMBS];
 };
 
 struct PassArgs {
@@ -103,7 +108,7 @@ struct TailArgs {
   int32_t first_round;
   int32_t num_rounds;
   int32_t challenge_mode;
-  int32_t pad0;
+  int32_t want_final;  // also derive the last challenge and fold the final point
   const CompDesc* comp;
   const FieldConsts* fc;
   ProofState* st;

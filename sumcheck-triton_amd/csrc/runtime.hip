@@ -225,6 +225,7 @@ struct sc_proof {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_mode = -1;
+  int last_mode = -1;  // challenge mode of the last run (the final point needs fixed or sha3)
   uint64_t device_bytes = 0;
   bool profiling = false;
   bool profiled_last = false;
@@ -330,7 +331,7 @@ int enqueue_rounds(sc_proof* p, int mode) {
     t.first_round = jt;
     t.num_rounds = p->num_vars;
     t.challenge_mode = mode;
-    t.pad0 = 0;
+    t.want_final = mode != SC_CHALLENGE_EXPLICIT ? 1 : 0;
     t.comp = p->d_comp;
     t.fc = fc;
     t.st = p->d_state;
@@ -565,6 +566,7 @@ int sc_proof_run(sc_proof* p, int32_t mode, const uint32_t* explicit_r) {
   rc = upload_explicit(p, mode, explicit_r);
   if (rc != SC_OK) return rc;
   hipStream_t s = p->ctx->stream;
+  p->last_mode = mode;
   SC_HIP(hipEventRecord(p->ev0, s));
   if (p->graph_exec != nullptr && p->graph_mode == mode && !p->profiling) {
     SC_HIP(hipGraphLaunch(p->graph_exec, s));
@@ -642,6 +644,26 @@ int sc_proof_result(sc_proof* p, uint32_t* round_polys, uint32_t* challenges, ui
     }
   }
   if (checksum) *checksum = sum;
+  return SC_OK;
+}
+
+int sc_proof_final(sc_proof* p, uint32_t* r_final, uint32_t* final_values) {
+  if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
+  SC_HIP(hipSetDevice(p->ctx->device));
+  SC_HIP(hipStreamSynchronize(p->ctx->stream));
+  std::vector<ProofState> host(1);
+  SC_HIP(hipMemcpy(host.data(), p->d_state, sizeof(ProofState), hipMemcpyDeviceToHost));
+  const ProofState& st = host[0];
+  if (st.final_ready == 0u || p->last_mode == SC_CHALLENGE_EXPLICIT)
+    return fail(SC_ERR_UNSUPPORTED, "no final point: run the proof with fixed or sha3 challenges first");
+  for (int i = 0; i < p->L; ++i) {
+    if (r_final) r_final[i] = st.r[p->num_vars - 1][i];
+  }
+  for (int v = 0; v < p->T; ++v) {
+    for (int i = 0; i < p->L; ++i) {
+      if (final_values) final_values[(size_t)v * p->L + i] = st.final_vals[v][i];
+    }
+  }
   return SC_OK;
 }
 

@@ -276,6 +276,16 @@ class Proof:
     def set_profiling(self, enable: bool = True) -> None:
         _lib.check(self.lib.sc_proof_set_profiling(self.handle, 1 if enable else 0))
 
+    def final(self) -> tuple[np.ndarray, np.ndarray]:
+        """Final-point step of the last run: (r_final [L], final_values [T][L]), Montgomery
+        limbs: the challenge derived from the last round polynomial and every table folded
+        at the full challenge point. Needs a run with fixed or sha3 challenges."""
+        L = self.field.limbs
+        r = np.zeros(L, dtype=np.uint32)
+        vals = np.zeros((self.comp.num_tables, L), dtype=np.uint32)
+        _lib.check(self.lib.sc_proof_final(self.handle, _lib.as_u32p(r), _lib.as_u32p(vals)))
+        return r, vals
+
     def kernel_times(self) -> tuple[list[float], float]:
         """(pass kernel ms per fused round, tail kernel ms) of the last profiled run."""
         rounds = self.info()["pass_rounds"]

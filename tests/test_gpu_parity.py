@@ -207,6 +207,59 @@ def test_wide_compositions_match_c_oracle(ctx, bw):
 
 
 @pytest.mark.parametrize("bw", WIDTHS)
+def test_final_point_and_verifier(ctx, bw):
+    """Final-point step: r_final follows the transcript and every table's final value is its
+    multilinear extension at (r_0, ..., r_final), computed here from the input tables; the
+    verifier accepts the whole proof, with claim, for compiled, interpreted and wide
+    compositions, in sha3 and fixed modes, under the default schedule and with every round
+    but the last outside the tail."""
+    from zkduel_hip.verify import verify
+
+    f = get_field(bw)
+    p = f.modulus
+    rng = random.Random(77 + bw)
+    wide = Composition.from_terms(10, [(rng.randrange(p), [rng.randrange(10) for _ in range(4)]) for _ in range(6)])
+    for comp, nv in ((Composition.from_spec(get_spec("poly_abc_plus_de")), 9),
+                     (Composition.from_spec(get_spec("zk_complete_add_3")), 7), (wide, 8),
+                     (Composition.from_spec(get_spec("poly_abc")), 1)):
+        with Proof(ctx, bw, comp, nv, input="uniform", seed=13) as pr:
+            tables = [[f.montgomery_decode(x) for x in f.limbs_to_ints(t)] for t in pr.tables()]
+            claim = sum(_eval_terms(comp, [t[i] for t in tables], p) for i in range(1 << nv)) % p
+            for mode, (blocks, tail) in (("sha3", (0, 0)), ("fixed", (0, 0)), ("sha3", (16, 1))):
+                say("final", bw, comp.num_tables, nv, mode, tail)
+                pr.set_schedule(blocks, tail, False)
+                pr.run(mode)
+                pr.wait()
+                g, r, _cs = pr.result()
+                rf, finals = pr.final()
+                verify(bw, comp, g, r, challenge_mode=mode, claim=claim, r_final=rf, final_values=finals)
+                point = [f.montgomery_decode(x) for x in f.limbs_to_ints(r)] + \
+                        [f.montgomery_decode(f.limbs_to_ints(rf.reshape(1, -1))[0])]
+                want = [_mle(t, point, p) for t in tables]
+                assert [f.montgomery_decode(x) for x in f.limbs_to_ints(finals)] == want
+
+
+def _eval_terms(comp, vals, p):
+    total = 0
+    for c, idx in comp.terms():
+        prod = c % p
+        for v in idx:
+            prod = prod * vals[v] % p
+        total += prod
+    return total % p
+
+
+def _mle(table, point, p):
+    total = 0
+    for i, x in enumerate(table):
+        w = 1
+        for k, r in enumerate(point):
+            w = w * (r if (i >> k) & 1 else 1 - r) % p
+        total = (total + x * w) % p
+    return total
+
+
+@pytest.mark.parametrize("bw", WIDTHS)
 def test_package_api_matches_oracle_reference_contract(bw):
     """tests/test_zkduel.py:54-82 of the reference, at every width: explicit challenges."""
     f = get_field(bw)
