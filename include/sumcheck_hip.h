@@ -53,6 +53,11 @@ typedef struct sc_composition {
 #define SC_INPUT_DEVICE_MONT 2    /* data: device tables, Montgomery form, borrowed */
 #define SC_INPUT_AFFINE 3         /* generated: seed + 7919(v+1) + 104729(o+1) (field_sumcheck.cu:123-128) */
 #define SC_INPUT_UNIFORM 4        /* generated: counter-based uniform in [0,p); selector tables in {0,1} */
+/* Limb-major tables, the reference's --layout limb order [table][limb][entry]
+ * (cuda/src/field_sumcheck.cu:165-190), Montgomery form: transposed once on the device into
+ * the element-major layout the kernels stream; results are the same bits. */
+#define SC_INPUT_HOST_MONT_LIMB 5   /* data: host tables */
+#define SC_INPUT_DEVICE_MONT_LIMB 6 /* data: device tables, read only */
 
 typedef struct sc_input {
   int32_t kind;

@@ -717,6 +717,26 @@ __global__ void __launch_bounds__(BLOCK) fill_kernel(FillArgs a) {
   }
 }
 
+// Limb-major tables [table][limb][entry] -> element-major [table][entry][limb] (input format
+// of the reference's --layout limb); n entries per table, a power of two.
+template <class F>
+__global__ void __launch_bounds__(BLOCK) limb_to_element_kernel(const uint32_t* __restrict__ in,
+                                                                uint32_t* __restrict__ out, uint64_t n,
+                                                                int num_tables) {
+  constexpr int L = F::L;
+  const int shift = __builtin_ctzll(n);
+  const uint64_t total = n * (uint64_t)num_tables;
+  for (uint64_t idx = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; idx < total;
+       idx += (uint64_t)gridDim.x * BLOCK) {
+    const uint64_t v = idx >> shift, i = idx & (n - 1);
+    uint32_t w[L];
+#pragma unroll
+    for (int k = 0; k < L; ++k) w[k] = in[(v * L + k) * n + i];
+#pragma unroll
+    for (int k = 0; k < L; ++k) out[idx * L + k] = w[k];
+  }
+}
+
 // Elementwise field operations, for the field-layer tests and host data conversion.
 // OP_WORK_MUL and OP_WORK_FOLD run the kernels' working-form path end to end: a*b, and
 // a + a (b - a) through fold, store and reload, each read back through the accumulator.

@@ -26,6 +26,8 @@ struct Launchers {
   // evaluation launch with pmax lanes per pair
   hipError_t (*wide_fold)(hipStream_t, const PassArgs&, unsigned grid);
   hipError_t (*wide_eval)(hipStream_t, int tmax, int pmax, const PassArgs&, unsigned grid);
+  // limb-major input tables -> element-major (n entries per table, a power of two)
+  hipError_t (*limb_to_element)(hipStream_t, const uint32_t* in, uint32_t* out, uint64_t n, int num_tables);
 };
 
 const Launchers& launchers_p32();

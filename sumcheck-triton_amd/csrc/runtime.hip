@@ -452,9 +452,10 @@ int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* co
   int degree = 0;
   int rc = build_comp_any(fidx, comp, ctx->host_fc[fidx], desc, degree);
   if (rc != SC_OK) return rc;
-  if (input->kind < SC_INPUT_HOST_CANONICAL || input->kind > SC_INPUT_UNIFORM)
+  if (input->kind < SC_INPUT_HOST_CANONICAL || input->kind > SC_INPUT_DEVICE_MONT_LIMB)
     return fail(SC_ERR_ARG, "unknown input kind");
-  if (input->kind <= SC_INPUT_DEVICE_MONT && input->data == nullptr)
+  const bool limb_major = input->kind == SC_INPUT_HOST_MONT_LIMB || input->kind == SC_INPUT_DEVICE_MONT_LIMB;
+  if ((input->kind <= SC_INPUT_DEVICE_MONT || limb_major) && input->data == nullptr)
     return fail(SC_ERR_ARG, "input data must not be NULL");
 
   SC_HIP(hipSetDevice(ctx->device));
@@ -520,6 +521,17 @@ int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* co
     if (input->kind == SC_INPUT_HOST_CANONICAL) {
       SC_HIP_P(p->ln->field_op(s, 3, (uint64_t)p->T * p->n0, p->d_tables, nullptr, p->d_tables, fc));
     }
+  } else if (input->kind == SC_INPUT_DEVICE_MONT_LIMB) {
+    SC_HIP_P(p->ln->limb_to_element(s, (const uint32_t*)input->data, p->d_tables, p->n0, p->T));
+  } else if (input->kind == SC_INPUT_HOST_MONT_LIMB) {
+    // staged through a temporary device buffer of the input's size, freed after the transpose
+    uint32_t* tmp = nullptr;
+    SC_HIP_P(hipMalloc(&tmp, words * 4));
+    hipError_t e = hipMemcpyAsync(tmp, input->data, words * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = p->ln->limb_to_element(s, tmp, p->d_tables, p->n0, p->T);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(tmp);
+    SC_HIP_P(e);
   } else if (input->kind == SC_INPUT_AFFINE || input->kind == SC_INPUT_UNIFORM) {
     FillArgs f;
     f.tables = p->d_tables;

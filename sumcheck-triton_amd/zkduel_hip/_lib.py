@@ -24,6 +24,8 @@ SC_INPUT_HOST_MONT = 1
 SC_INPUT_DEVICE_MONT = 2
 SC_INPUT_AFFINE = 3
 SC_INPUT_UNIFORM = 4
+SC_INPUT_HOST_MONT_LIMB = 5
+SC_INPUT_DEVICE_MONT_LIMB = 6
 
 SC_CHALLENGE_FIXED = 0
 SC_CHALLENGE_SHA3 = 1

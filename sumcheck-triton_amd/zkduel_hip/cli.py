@@ -7,8 +7,10 @@ experiments/h100/tools/validate_cuda_vs_registry.py:66-91).
     python -m zkduel_hip.cli --workloads poly_abc --bit-widths 64 --rounds 10 16 \\
         --warmups 0 --repeats 1 --challenge-mode sha3 --out results.csv
 
-Differences: --layout accepts only "element"; --block is accepted and ignored (the pass grid
-is chosen per field); errors exit with status 2 (bad flags) or 1 (runtime), as the reference.
+Differences: --layout limb hands the generated tables over limb-major ([table][limb][entry])
+and the library transposes them once on the device, so the kernels always stream
+element-major; --block is accepted and ignored (the pass grid is chosen per field); errors exit
+with status 2 (bad flags) or 1 (runtime), as the reference.
 """
 
 from __future__ import annotations
@@ -87,8 +89,6 @@ def parse_args(argv: list[str]) -> dict:
         i += 1
     if a["block"] <= 0 or a["block"] & (a["block"] - 1):
         _die("--block must be a positive power of two")
-    if a["layout"] != "element":
-        _die("--layout limb is not implemented by the HIP library")
     return a
 
 

@@ -69,8 +69,8 @@ def time_one(
         raise ValueError("timing_mode must be whole-run or round-sum")
     if challenge_mode not in ("fixed", "sha3"):
         raise ValueError("challenge_mode must be fixed or sha3")
-    if layout != "element":
-        raise ValueError("only the element layout is implemented")
+    if layout not in ("element", "limb"):
+        raise ValueError("layout must be element or limb")
     if point_mode not in ("generic", "specialized"):
         raise ValueError("point_mode must be generic or specialized")
     field = get_field(bit_width)
@@ -79,7 +79,15 @@ def time_one(
     ctx = ctx or default_context()
     n = 1 << rounds
     in_seed = seed + rounds if input_mode == "affine" else seed
-    with Proof(ctx, bit_width, comp, rounds, input=input_mode, seed=in_seed) as proof:
+    if layout == "limb":
+        # the reference's limb-major tables ([table][limb][entry], field_sumcheck.cu:165-190):
+        # the same generated inputs handed over limb-major and transposed on the device
+        with Proof(ctx, bit_width, comp, rounds, input=input_mode, seed=in_seed) as gen:
+            limb_major = np.ascontiguousarray(gen.tables().transpose(0, 2, 1))
+        source = {"input": "host-mont-limb", "data": limb_major}
+    else:
+        source = {"input": input_mode, "seed": in_seed}
+    with Proof(ctx, bit_width, comp, rounds, **source) as proof:
         first_ms = run_timed(proof, challenge_mode, timer)
         _p, _c, checksum = proof.result()
         for _ in range(warmups):

@@ -24,6 +24,8 @@ INPUT_KINDS = {
     "device-mont": _lib.SC_INPUT_DEVICE_MONT,
     "affine": _lib.SC_INPUT_AFFINE,
     "uniform": _lib.SC_INPUT_UNIFORM,
+    "host-mont-limb": _lib.SC_INPUT_HOST_MONT_LIMB,
+    "device-mont-limb": _lib.SC_INPUT_DEVICE_MONT_LIMB,
 }
 
 
@@ -181,14 +183,14 @@ class Proof:
         kind = INPUT_KINDS[input]
         self._keep = None
         ptr = None
-        if kind in (_lib.SC_INPUT_HOST_CANONICAL, _lib.SC_INPUT_HOST_MONT):
+        if kind in (_lib.SC_INPUT_HOST_CANONICAL, _lib.SC_INPUT_HOST_MONT, _lib.SC_INPUT_HOST_MONT_LIMB):
             arr = np.ascontiguousarray(data, dtype=np.uint32)
             expect = comp.num_tables * (1 << self.num_vars) * self.field.limbs
             if arr.size != expect:
                 raise ValueError(f"input has {arr.size} words, expected {expect}")
             self._keep = arr
             ptr = arr.ctypes.data
-        elif kind == _lib.SC_INPUT_DEVICE_MONT:
+        elif kind in (_lib.SC_INPUT_DEVICE_MONT, _lib.SC_INPUT_DEVICE_MONT_LIMB):
             ptr = int(data)
         inp = _lib.ScInput(kind, int(instance) & 0xFFFFFFFF, ptr, int(seed) & ((1 << 64) - 1),
                            int(selector_mask) & ((1 << 64) - 1))

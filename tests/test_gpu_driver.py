@@ -42,3 +42,56 @@ def test_time_one_row_and_checksum_multiplicity():
     assert row["checksum"] == 6 * one  # first + 2 warmups + 3 repeats
     assert row["vars"] == 11 and row["degree"] == 5 and row["N"] == 1024
     assert row["round_polys"].shape == (10, 6, 4)
+
+
+def test_limb_major_layout_matches_registry_goldens(tmp_path):
+    """--layout limb (the reference's [table][limb][entry] tables, handed over limb-major and
+    transposed on the device) reproduces the same recorded checksums."""
+    out = tmp_path / "hip_limb.csv"
+    rc = cli_main(["--workloads", "poly_abc_plus_de", "zk_complete_add_3", "--bit-widths", "64", "256",
+                   "--rounds", "10", "--warmups", "0", "--repeats", "1", "--challenge-mode", "sha3",
+                   "--layout", "limb", "--out", str(out)])
+    assert rc == 0
+    rows = list(csv.DictReader(out.open()))
+    assert len(rows) == 4
+    for row in rows:
+        assert row["layout"] == "limb"
+        key = (row["workload"], int(row["bit_width"]), int(row["rounds"]), "sha3")
+        assert int(row["checksum"]) == REGISTRY[key], key
+
+
+def test_limb_major_inputs_give_identical_proofs():
+    """Host and device limb-major inputs give the bits of the element-major proof."""
+    import ctypes
+
+    import numpy as np
+
+    from zkduel_hip import Composition, Context, Proof, get_spec
+
+    ctx = Context(0)
+    hip = ctypes.CDLL("libamdhip64.so")  # already loaded by the library; synchronous copies only
+    comp = Composition.from_spec(get_spec("poly_abc_plus_de"))
+    for bw in (32, 64, 128, 256):
+        with Proof(ctx, bw, comp, 12, input="uniform", seed=21) as pr:
+            pr.run("sha3")
+            pr.wait()
+            want = pr.result()
+            limb_major = np.ascontiguousarray(pr.tables().transpose(0, 2, 1))
+        with Proof(ctx, bw, comp, 12, input="host-mont-limb", data=limb_major) as pr:
+            pr.run("sha3")
+            pr.wait()
+            got = pr.result()
+        assert np.array_equal(got[0], want[0]) and got[2] == want[2]
+        dev = ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(dev), ctypes.c_size_t(limb_major.nbytes)) == 0
+        try:
+            assert hip.hipMemcpy(dev, limb_major.ctypes.data_as(ctypes.c_void_p),
+                                 ctypes.c_size_t(limb_major.nbytes), 1) == 0  # host to device
+            with Proof(ctx, bw, comp, 12, input="device-mont-limb", data=dev.value) as pr:
+                pr.run("sha3")
+                pr.wait()
+                got = pr.result()
+        finally:
+            hip.hipFree(dev)
+        assert np.array_equal(got[0], want[0]) and got[2] == want[2]
+    ctx.close()

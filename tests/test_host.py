@@ -147,7 +147,7 @@ def test_cli_flag_grammar_matches_reference_executable():
     assert a["rounds"] == [3, 10] and a["bit_widths"] == [64, 256]
     assert a["workloads"] == ["zkphire_8", "poly_abc"] and a["zk_only"]
     assert (a["warmups"], a["repeats"], a["challenge_mode"], a["point_mode"]) == (0, 1, "sha3", "specialized")
-    for bad in (["--block", "3"], ["--timing-mode", "x"], ["--nope"], ["--layout", "limb"], ["--seed"]):
+    for bad in (["--block", "3"], ["--timing-mode", "x"], ["--nope"], ["--layout", "diag"], ["--seed"]):
         with pytest.raises(SystemExit) as e:
             parse_args(bad)
         assert e.value.code == 2
