@@ -146,6 +146,22 @@ __device__ __forceinline__ void compose_static(typename Work<F>::Acc& acc, const
                                                const CompDesc* __restrict__ cd) {
   using W = Work<F>;
   static_assert(STATIC_COMPS[SID].num_tables <= TMAX, "table slots");
+  constexpr StaticComp C2 = STATIC_COMPS[SID];
+  if constexpr (C2.num_terms == 2 && C2.kinds[0] == COEFF_ONE && C2.kinds[1] == COEFF_ONE && C2.lens[0] >= 2 &&
+                C2.lens[1] >= 2) {
+    // Two products with unit coefficients: their last multiplications share one reduction.
+    typename W::V pre[2];
+    static_for<2>([&](auto ki) {
+      constexpr int k = decltype(ki)::value;
+      pre[k] = x[C2.vars[k][0]];
+      static_for<C2.lens[k] - 2>([&](auto mi) {
+        constexpr int m = decltype(mi)::value + 1;
+        pre[k] = W::mul(pre[k], x[STATIC_COMPS[SID].vars[k][m]]);
+      });
+    });
+    acc.add(W::mul_sum(pre[0], x[C2.vars[0][C2.lens[0] - 1]], pre[1], x[C2.vars[1][C2.lens[1] - 1]]));
+    return;
+  }
   static_for<STATIC_COMPS[SID].num_terms>([&](auto ki) {
     constexpr int k = decltype(ki)::value;
     constexpr StaticComp C = STATIC_COMPS[SID];

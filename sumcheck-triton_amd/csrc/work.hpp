@@ -9,6 +9,7 @@
 //   store(w, x)       working value -> fold-buffer words (same size as a Montgomery element)
 //   coeff(w)          composition coefficient, already in working form on the host (work_coeff)
 //   mul(a, b)         product
+//   mul_sum(a1, b1, a2, b2)  a1 b1 + a2 b2 (one reduction where the form allows it)
 //   diff(hi, lo)      hi - lo, the interpolation step of a pair
 //   step(x, d)        x + d
 //   fold(lo, hi, r)   lo + r (hi - lo), the folded entry, ready for store
@@ -43,6 +44,10 @@ struct Work {
     for (int i = 0; i < L; ++i) w[i] = x.v[i];
   }
   SC_HD static V mul(const V& a, const V& b) { return mont_mul<F>(a, b); }
+  // a1 b1 + a2 b2
+  SC_HD static V mul_sum(const V& a1, const V& b1, const V& a2, const V& b2) {
+    return step_add<F>(mont_mul<F>(a1, b1), mont_mul<F>(a2, b2));
+  }
   SC_HD static V diff(const V& hi, const V& lo) { return sub<F>(hi, lo); }
   SC_HD static V step(const V& x, const V& d) { return step_add<F>(x, d); }
   SC_HD static V fold(const V& lo, const V& hi, const V& r) { return add<F>(lo, mont_mul<F>(r, sub<F>(hi, lo))); }
@@ -168,30 +173,62 @@ SC_HD R29 r29_mul(const R29& a, const R29& b) {
   uint32_t m[9];
   R29 r;
   uint64_t acc = 0;
+  // One multiply-add chain per column (independent products of the caller interleave with
+  // it); m_k * p_0 = m_k since p_0 = 1.
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    uint64_t s1 = acc + (uint64_t)a.l[k] * b.l[0];
-    uint64_t s2 = 0;
+    uint64_t s = acc;
 #pragma unroll
-    for (int i = 0; i < k; ++i) {
-      s1 += (uint64_t)a.l[i] * b.l[k - i];
-      s2 += (uint64_t)m[i] * r29::P(k - i);
-    }
-    uint64_t s = s1 + s2;
+    for (int i = 0; i <= k; ++i) s += (uint64_t)a.l[i] * b.l[k - i];
+#pragma unroll
+    for (int i = 0; i < k; ++i) s += (uint64_t)m[i] * r29::P(k - i);
     m[k] = (0u - (uint32_t)s) & r29::MASK;
-    s += m[k];
-    acc = s >> 29;
+    acc = (s + m[k]) >> 29;
   }
 #pragma unroll
   for (int k = 9; k < 17; ++k) {
-    uint64_t s1 = acc;
-    uint64_t s2 = 0;
+    uint64_t s = acc;
 #pragma unroll
     for (int i = k - 8; i < 9; ++i) {
-      s1 += (uint64_t)a.l[i] * b.l[k - i];
-      s2 += (uint64_t)m[i] * r29::P(k - i);
+      s += (uint64_t)a.l[i] * b.l[k - i];
+      s += (uint64_t)m[i] * r29::P(k - i);
     }
-    const uint64_t s = s1 + s2;
+    r.l[k - 9] = (uint32_t)s & r29::MASK;
+    acc = s >> 29;
+  }
+  r.l[8] = (uint32_t)acc;
+  return r;
+}
+
+// (a1 b1 + a2 b2) * 2^-261 mod p with one reduction: both products summed per column (18
+// products, 8 of m*p, m_k and the carry: below 2^62.7 for limbs < 2^29). Value below
+// (a1 b1 + a2 b2)/2^261 + p.
+SC_HD R29 r29_mul_sum(const R29& a1, const R29& b1, const R29& a2, const R29& b2) {
+  uint32_t m[9];
+  R29 r;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    uint64_t s = acc;
+#pragma unroll
+    for (int i = 0; i <= k; ++i) {
+      s += (uint64_t)a1.l[i] * b1.l[k - i];
+      s += (uint64_t)a2.l[i] * b2.l[k - i];
+    }
+#pragma unroll
+    for (int i = 0; i < k; ++i) s += (uint64_t)m[i] * r29::P(k - i);
+    m[k] = (0u - (uint32_t)s) & r29::MASK;
+    acc = (s + m[k]) >> 29;
+  }
+#pragma unroll
+  for (int k = 9; k < 17; ++k) {
+    uint64_t s = acc;
+#pragma unroll
+    for (int i = k - 8; i < 9; ++i) {
+      s += (uint64_t)a1.l[i] * b1.l[k - i];
+      s += (uint64_t)a2.l[i] * b2.l[k - i];
+      s += (uint64_t)m[i] * r29::P(k - i);
+    }
     r.l[k - 9] = (uint32_t)s & r29::MASK;
     acc = s >> 29;
   }
@@ -264,6 +301,7 @@ struct Work<FieldP256> {
   SC_HD static V coeff(const uint32_t* w) { return r29_unpack<0>(w); }
   SC_HD static void store(uint32_t* w, const V& x) { r29_pack<0>(w, x); }
   SC_HD static V mul(const V& a, const V& b) { return r29_mul(a, b); }
+  SC_HD static V mul_sum(const V& a1, const V& b1, const V& a2, const V& b2) { return r29_mul_sum(a1, b1, a2, b2); }
   // hi + 3p - lo for hi, lo < p + 2^241: value < 4p + 2^241.
   SC_HD static V diff(const V& hi, const V& lo) {
     R29 d;
@@ -370,16 +408,17 @@ SC_HD void r26_pack(uint32_t* w, const R26& x) {
   w[3] = (uint32_t)acc;
 }
 
-// a * b mod p (lazily reduced): ten columns, the upper five folded down with 2^130 = 8.
-SC_HD R26 r26_mul(const R26& a, const R26& b) {
-  uint64_t c[9];
+// The nine column sums of a * b (each below 2^57 for limbs below 2^27).
+SC_HD void r26_columns(uint64_t (&c)[9], const R26& a, const R26& b) {
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    uint64_t s = 0;
 #pragma unroll
-    for (int i = (k < 5 ? 0 : k - 4); i <= (k < 5 ? k : 4); ++i) s += (uint64_t)a.l[i] * b.l[k - i];
-    c[k] = s;
+    for (int i = (k < 5 ? 0 : k - 4); i <= (k < 5 ? k : 4); ++i) c[k] += (uint64_t)a.l[i] * b.l[k - i];
   }
+}
+
+// Column sums -> lazily reduced residue: the upper five columns folded down with 2^130 = 8.
+SC_HD R26 r26_reduce_columns(uint64_t (&c)[9]) {
 #pragma unroll
   for (int k = 0; k < 4; ++k) c[k] += c[k + 5] << 3;
   R26 r;
@@ -397,6 +436,21 @@ SC_HD R26 r26_mul(const R26& a, const R26& b) {
   r.l[0] += r.l[4] >> 23;
   r.l[4] &= (1u << 23) - 1;
   return r;
+}
+
+// a * b mod p (lazily reduced).
+SC_HD R26 r26_mul(const R26& a, const R26& b) {
+  uint64_t c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  r26_columns(c, a, b);
+  return r26_reduce_columns(c);
+}
+
+// a1 b1 + a2 b2 mod p with one reduction (column sums below 2^58).
+SC_HD R26 r26_mul_sum(const R26& a1, const R26& b1, const R26& a2, const R26& b2) {
+  uint64_t c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  r26_columns(c, a1, b1);
+  r26_columns(c, a2, b2);
+  return r26_reduce_columns(c);
 }
 
 // Plain residue (value below 2^127 + 2^5) -> Montgomery limbs 2x mod p in [0, p).
@@ -443,6 +497,7 @@ struct Work<FieldP128> {
   SC_HD static V coeff(const uint32_t* w) { return r26_unpack(w); }
   SC_HD static void store(uint32_t* w, const V& x) { r26_pack(w, x); }
   SC_HD static V mul(const V& a, const V& b) { return r26_mul(a, b); }
+  SC_HD static V mul_sum(const V& a1, const V& b1, const V& a2, const V& b2) { return r26_mul_sum(a1, b1, a2, b2); }
   SC_HD static V diff(const V& hi, const V& lo) {  // hi + 2p - lo
     R26 d;
 #pragma unroll

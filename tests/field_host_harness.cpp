@@ -40,6 +40,7 @@ extern "C" void field_host_op(int bit_width, int op, uint64_t n, const uint32_t*
 //   op 2: -(c * x^3) with x = a + 7 (b - a) reached by seven interpolation steps
 //   op 3: sum of 40 products a*b minus one, through the accumulator (39 a*b)
 //   op 4: a*b - c*x(t) for every t in 0..7, summed (x(t) = a + t (b - a))
+//   op 5: a x^2 + c a, x = a + 7 (b - a), the last two products sharing one reduction
 template <class F>
 static void run_work(int op, uint64_t n, const uint32_t* a, const uint32_t* b, const uint32_t* c, uint32_t* out) {
   using W = Work<F>;
@@ -63,6 +64,11 @@ static void run_work(int op, uint64_t n, const uint32_t* a, const uint32_t* b, c
       const typename W::V pr = W::mul(x, y);
       for (int k = 0; k < 40; ++k) acc.add(pr);
       acc.sub(pr);
+    } else if (op == 5) {
+      const typename W::V d = W::diff(y, x);
+      typename W::V t = y;
+      for (int k = 2; k <= 7; ++k) t = W::step(t, d);
+      acc.add(W::mul_sum(W::mul(x, t), t, z, x));  // a x7^2 + c a with x7 = a + 7 (b - a)
     } else {
       const typename W::V d = W::diff(y, x);
       typename W::V t = x;

@@ -109,6 +109,7 @@ def test_host_working_form(harness, bw):
         xs = sum((x + t * (y - x)) for t in range(8))
         want4.append((8 * mont(x, y) - mont(z, xs % p)) % p)
     assert harness.work(bw, 4, a, b, c) == want4
+    assert harness.work(bw, 5, a, b, c) == [(mont(mont(x, t), t) + mont(z, x)) % p for x, t, z in zip(a, x7, c)]
 
 
 def test_host_r29_pack_and_reduce(harness):
