@@ -315,20 +315,16 @@ __device__ __forceinline__ void block_reduce(Elem<F::L> (&acc)[PMAX], int npts, 
 }
 
 // SHA3-256 of a padded message held in LDS, run by one whole wave (lane k of the first 25
-// holds state word k); the digest is returned to every lane.
-__device__ __forceinline__ void sha3_uniform(const uint32_t* msg, int nblocks, uint32_t digest[8]) {
+// holds state word k). Returns digest word (lane mod 8) to every lane.
+__device__ __forceinline__ uint32_t sha3_wave_word(const uint32_t* msg, int nblocks) {
   const int lane = threadIdx.x & 63;
   uint64_t a = 0;
   for (int b = 0; b < nblocks; ++b) {
     if (lane < 17) a ^= (uint64_t)msg[34 * b + 2 * lane] | ((uint64_t)msg[34 * b + 2 * lane + 1] << 32);
     keccak_f1600_wave(a);
   }
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const uint64_t w = shfl64(a, i);
-    digest[2 * i] = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)w);
-    digest[2 * i + 1] = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(w >> 32));
-  }
+  const uint64_t w = shfl64(a, (lane & 7) >> 1);
+  return (lane & 1) ? (uint32_t)(w >> 32) : (uint32_t)w;
 }
 
 // Publishes g (LDS, npts*L words) as round `round`'s polynomial and, unless it is the last
@@ -351,14 +347,20 @@ __device__ __forceinline__ void finish_round(const uint32_t* g, int npts, int ro
     for (int k = tid; k < 34 * nb; k += NT) msg[k] = transcript_word(k, (uint32_t)round, g, nvals, nb);
     __syncthreads();
     if (tid < 64) {
-      uint32_t digest[8];
-      sha3_uniform(msg, nb, digest);
+      // lane i < 8 multiplies digest word i by its power, then the eight terms are summed
+      const uint32_t word = sha3_wave_word(msg, nb);
       Elem<L> r = zero<F>();
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
+      if (tid < 8) {
         Elem<L> w = zero<F>();
-        w.v[0] = digest[i];
-        r = add<F>(r, mont_mul<F>(w, load_elem<L>(fc->word_pow[i])));
+        w.v[0] = word;
+        r = mont_mul<F>(w, load_elem<L>(fc->word_pow[tid]));
+      }
+#pragma unroll
+      for (int off = 1; off < 8; off <<= 1) {
+        Elem<L> o;
+#pragma unroll
+        for (int k = 0; k < L; ++k) o.v[k] = __shfl_xor(r.v[k], off, 64);
+        r = add<F>(r, o);
       }
       if (tid == 0) {
 #pragma unroll

@@ -236,12 +236,14 @@ namespace {
 
 // Schedule defaults per field, from sweeps on MI355X (a*b*c + d*e at num_vars 24-26): the
 // pass grid cap, and the pair count at which the remaining rounds move to the tail kernel.
+// Pass-kernel grid caps from sweeps on MI355X (a*b*c + d*e at the benchmark sizes; fewer
+// blocks also means fewer partials for the finalize kernel to sum).
 int default_pass_blocks(int L) {
   switch (L) {
-    case 1: return 4096;
+    case 1: return 2048;
     case 2: return 1024;
     case 4: return 2048;
-    default: return 1024;
+    default: return 512;
   }
 }
 
