@@ -202,26 +202,63 @@ def measure_serving(ctx, rate_per_s: float, n: int, budget_bytes: int = 200 * 10
     return out
 
 
+def _oracle_time(bw: int, terms, T: int, nv: int, selector_mask: int = 0) -> float:
+    from oracle import sumcheck_ref as R
+
+    p, L = R.field(bw)
+    sel = tuple(v for v in range(T) if (selector_mask >> v) & 1)
+    tables = R.uniform_tables(T, nv, SEED, p, L, selectors=sel)
+    t0 = time.perf_counter()
+    R.prove(tables, terms, p, nv, R.sha3_challenger(p, L))
+    return time.perf_counter() - t0
+
+
+def _host_info() -> dict:
+    model = ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "python": sys.version.split()[0]}
+
+
 def cpu_baseline(budget_s: float = 20.0) -> dict:
     """Oracle (single-core Python restatement of reference.py) on the same workload at the
     largest num_vars that fits the budget; proofs/s scaled to num_vars 26 by 2^(nv - 26)
-    (the prover's work is linear in the table size)."""
-    from oracle import sumcheck_ref as R
+    (the prover's work is linear in the table size). Also timed, on one core each: every
+    width of a*b*c + d*e at num_vars 16 (scaled to its benchmark size), the Goldilocks a*b*c
+    proof at num_vars 16 in full, and the BLS12-381 constraint workload at num_vars 12
+    (scaled to 22)."""
     from zkduel_hip import get_spec
+    from zkduel_hip.synthetic import constraint_composition
 
-    p, L = R.field(64)
     spec = get_spec(WORKLOAD)
     terms = [(c, list(spec.term_vars[o : o + n])) for c, n, o in zip(spec.coeffs, spec.term_lens, spec.term_offsets)]
     nv, secs = 8, 0.0
     while True:
-        tables = R.uniform_tables(spec.num_vars, nv, SEED, p, L)
-        t0 = time.perf_counter()
-        R.prove(tables, terms, p, nv, R.sha3_challenger(p, L))
-        secs = time.perf_counter() - t0
+        secs = _oracle_time(64, terms, spec.num_vars, nv)
         if secs * 2.2 > budget_s or nv >= 20:
             break
         nv += 1
     full = SIZES[64]
+    per = {}
+    for bw in (32, 64, 128, 256):
+        t = _oracle_time(bw, terms, spec.num_vars, 16)
+        per[f"abc_plus_de_{bw}bit"] = {"num_vars_timed": 16, "seconds": round(t, 3),
+                                       "proofs_per_s_at_benchmark_size": (1.0 / t) * 2.0 ** (16 - SIZES[bw]),
+                                       "benchmark_num_vars": SIZES[bw]}
+    abc = get_spec("poly_abc")
+    abc_terms = [(c, list(abc.term_vars[o : o + n])) for c, n, o in zip(abc.coeffs, abc.term_lens, abc.term_offsets)]
+    t = _oracle_time(64, abc_terms, abc.num_vars, 16)
+    per["goldilocks_abc_nv16"] = {"num_vars_timed": 16, "seconds": round(t, 3), "proofs_per_s": 1.0 / t}
+    comp3, mask3 = constraint_composition(256)
+    t = _oracle_time(256, list(comp3.terms()), comp3.num_tables, 12, mask3)
+    per["bls_constraint_12_monomials"] = {"num_vars_timed": 12, "seconds": round(t, 3),
+                                          "proofs_per_s_at_benchmark_size": (1.0 / t) * 2.0 ** (12 - 22),
+                                          "benchmark_num_vars": 22}
     return {
         "value": (1.0 / secs) * 2.0 ** (nv - full),
         "unit": "proofs/s",
@@ -229,6 +266,9 @@ def cpu_baseline(budget_s: float = 20.0) -> dict:
         "kind": "port",
         "sample": f"oracle/sumcheck_ref.py prove(), Goldilocks a*b*c+d*e, 5 uniform tables, SHA3, "
                   f"num_vars={nv} took {secs:.2f} s on 1 core; scaled by 2^({nv}-{full})",
+        "per_workload": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                         for k, v in per.items()},
+        "host": _host_info(),
     }
 
 
