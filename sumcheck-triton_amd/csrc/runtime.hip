@@ -237,8 +237,11 @@ namespace {
 // Schedule defaults per field, from sweeps on MI355X (a*b*c + d*e at num_vars 24-26): the
 // pass grid cap, and the pair count at which the remaining rounds move to the tail kernel.
 // Pass-kernel grid caps from sweeps on MI355X (a*b*c + d*e at the benchmark sizes; fewer
-// blocks also means fewer partials for the finalize kernel to sum).
-int default_pass_blocks(int L) {
+// blocks also means fewer partials for the finalize kernel to sum). Compositions of more than
+// 8 tables use the point-parallel kernel, which holds few registers per lane and wants the
+// largest grid (BLS12-381 constraint workload: 22.9 ms at 512 blocks, 19.4 ms at 4096).
+int default_pass_blocks(int L, int tmax) {
+  if (tmax > 8) return MAX_PASS_BLOCKS;
   switch (L) {
     case 1: return 2048;
     case 2: return 1024;
@@ -474,7 +477,7 @@ int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* co
   p->pmax = p->npts <= 4 ? 4 : 8;
   p->n0 = (uint64_t)1 << num_vars;
   p->tail_pairs = default_tail_pairs(p->L, p->tmax);
-  p->max_pass_blocks = default_pass_blocks(p->L);
+  p->max_pass_blocks = default_pass_blocks(p->L, p->tmax);
   p->matched_sid = match_static(desc);
   p->sid = p->matched_sid;
 
@@ -560,7 +563,7 @@ int sc_proof_set_schedule(sc_proof* p, int32_t max_pass_blocks, int64_t tail_pai
   if (max_pass_blocks < 0 || max_pass_blocks > MAX_PASS_BLOCKS)
     return fail(SC_ERR_ARG, "max_pass_blocks must be in [0, 4096]");
   if (tail_pairs < 0) return fail(SC_ERR_ARG, "tail_pairs must be >= 0");
-  p->max_pass_blocks = max_pass_blocks ? max_pass_blocks : default_pass_blocks(p->L);
+  p->max_pass_blocks = max_pass_blocks ? max_pass_blocks : default_pass_blocks(p->L, p->tmax);
   p->tail_pairs = tail_pairs ? tail_pairs : default_tail_pairs(p->L, p->tmax);
   if (p->graph_exec) {
     (void)hipGraphExecDestroy(p->graph_exec);
