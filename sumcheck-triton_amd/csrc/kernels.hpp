@@ -211,7 +211,7 @@ struct Shape {
 template <class F, int TMAX, int PMAX, int SID>
 __device__ __forceinline__ void eval_points(typename Work<F>::V (&cur)[TMAX], typename Work<F>::V (&oth)[TMAX],
                                             typename Work<F>::Acc (&acc)[PMAX], const CompDesc* __restrict__ cd,
-                                            int T, int K, int npts) {
+                                            int T, int K, int npts, bool skip1) {
   using W = Work<F>;
   constexpr bool X = SID != 0 || TMAX <= 8;  // expand every loop unless the table count is large
   compose<F, SID, TMAX>(acc[0], cur, cd, K);
@@ -223,7 +223,7 @@ __device__ __forceinline__ void eval_points(typename Work<F>::V (&cur)[TMAX], ty
       cur[v] = o;
     }
   });
-  compose<F, SID, TMAX>(acc[1], cur, cd, K);
+  if (!skip1) compose<F, SID, TMAX>(acc[1], cur, cd, K);  // else finish_round derives g(1)
   table_for<PMAX - 2, X>([&](auto ti) {
     const int t = (int)ti + 2;
     if (t < npts) {
@@ -330,13 +330,58 @@ __device__ __forceinline__ uint32_t sha3_wave_word(const uint32_t* msg, int nblo
 // Publishes g (LDS, npts*L words) as round `round`'s polynomial and, unless it is the last
 // round (or last_challenge asks for the final-point challenge too), derives r_round into
 // r_out (LDS, L words) and st->r[round]. All threads call it.
+// The claim of round `round` >= 1: g_{round-1}(r_{round-1}) by barycentric interpolation
+// from the previous polynomial's values at 0..npts-1, one point per lane of the calling wave,
+// in the field's working form. Returns the canonical Montgomery value to every lane.
+template <class F>
+__device__ __forceinline__ Elem<F::L> previous_claim(const ProofState* st, int round, int npts,
+                                                     const FieldConsts* __restrict__ fc) {
+  using W = Work<F>;
+  constexpr int L = F::L;
+  const int t = threadIdx.x & 63;
+  Elem<L> term = zero<F>();
+  if (t < npts) {
+    const typename W::V r = W::from_mont(st->r[round - 1]);
+    typename W::V prod = W::mul(W::from_mont(st->polys[round - 1][t]), W::from_mont(fc->bary[npts - 1][t]));
+    for (int s = 0; s < npts; ++s) {
+      if (s != t) prod = W::mul(prod, W::diff(r, W::from_mont(fc->small[s])));
+    }
+    typename W::Acc acc;
+    acc.clear();
+    acc.add(prod);
+    term = acc.value();
+  }
+#pragma unroll
+  for (int off = 1; off < MAX_POINTS; off <<= 1) {
+    Elem<L> o;
+#pragma unroll
+    for (int k = 0; k < L; ++k) o.v[k] = __shfl_xor(term.v[k], off, 64);
+    term = add<F>(term, o);
+  }
+  return term;
+}
+
+// With fill_g1 (every prover path; the per-pair kernels skip point 1 from round 1 on),
+// g(1) of round >= 1 is set to claim - g(0): the protocol's own identity
+// g_j(0) + g_j(1) = g_{j-1}(r_{j-1}), so the value is the one evaluation would give.
 template <class F, int NT>
-__device__ __forceinline__ void finish_round(const uint32_t* g, int npts, int round, int num_rounds, int mode,
+__device__ __forceinline__ void finish_round(uint32_t* g, int npts, int round, int num_rounds, int mode,
                              const FieldConsts* __restrict__ fc, ProofState* st,
                              const uint32_t* __restrict__ explicit_r, uint32_t* msg, uint32_t* r_out,
-                             bool last_challenge = false) {
+                             bool last_challenge = false, bool fill_g1 = false) {
   constexpr int L = F::L;
   const int tid = threadIdx.x;
+  if (fill_g1 && round > 0) {
+    if (tid < 64) {
+      const Elem<L> claim = previous_claim<F>(st, round, npts, fc);
+      if (tid == 0) {
+        const Elem<L> g1 = sub<F>(claim, load_elem<L>(g));
+#pragma unroll
+        for (int k = 0; k < L; ++k) g[L + k] = g1.v[k];
+      }
+    }
+    __syncthreads();
+  }
   if (tid < npts * L) st->polys[round][tid / L][tid % L] = g[tid];
   if (round + 1 >= num_rounds && !last_challenge) return;
   if (mode == CHALLENGE_SHA3) {
@@ -404,7 +449,7 @@ __device__ __forceinline__ void reduce_partials_and_finish(const uint32_t* parti
     }
   }
   block_reduce<F, PMAX, BLOCK>(acc, npts, s_red, s_g);
-  finish_round<F, BLOCK>(s_g, npts, round, num_rounds, mode, fc, st, explicit_r, s_msg, s_r);
+  finish_round<F, BLOCK>(s_g, npts, round, num_rounds, mode, fc, st, explicit_r, s_msg, s_r, false, true);
 }
 
 template <class F, int TMAX, int PMAX, int SID>
@@ -432,7 +477,7 @@ __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
   for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.pairs; i += stride) {
     typename W::V cur[TMAX], oth[TMAX];
     load_pair<F, TMAX>(cur, oth, a.src, a.dst, a.src_n, i, T, fold, src_mont, r);
-    eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts);
+    eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts, fold);
   }
   Elem<L> sums[PMAX];
 #pragma unroll
@@ -499,14 +544,14 @@ __global__ void __launch_bounds__(TB) tail_kernel(TailArgs a) {
     for (uint64_t i = threadIdx.x; i < pairs; i += TB) {
       typename W::V cur[TMAX], oth[TMAX];
       load_pair<F, TMAX>(cur, oth, src, dst, src_n, i, T, fold, j <= 1, r);
-      eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts);
+      eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts, fold);
     }
     Elem<L> sums[PMAX];
 #pragma unroll
     for (int t = 0; t < PMAX; ++t) sums[t] = acc[t].value();
     block_reduce<F, PMAX, TB>(sums, npts, s_red, s_g);
     finish_round<F, TB>(s_g, npts, j, a.num_rounds, a.challenge_mode, a.fc, a.st, a.explicit_r, s_msg, s_r,
-                        a.want_final != 0);
+                        a.want_final != 0, true);
     __syncthreads();
   }
   if (!a.want_final) return;
@@ -672,7 +717,7 @@ __global__ void __launch_bounds__(BLOCK) batch_pass_kernel(BatchArgs a) {
     if (i < pairs) {
       typename W::V cur[TMAX], oth[TMAX];
       load_pair<F, TMAX>(cur, oth, src, dst, src_n, i, T, fold, j <= 1, r);
-      eval_points<F, TMAX, PMAX, 0>(cur, oth, acc, cd, T, K, npts);
+      eval_points<F, TMAX, PMAX, 0>(cur, oth, acc, cd, T, K, npts, fold);
     }
   }
   Elem<L> sums[PMAX];
@@ -707,7 +752,7 @@ __global__ void __launch_bounds__(BLOCK) batch_finalize_kernel(BatchArgs a) {
   }
   block_reduce<F, PMAX, BLOCK>(acc, npts, s_red, s_g);
   finish_round<F, BLOCK>(s_g, npts, (int)ent.round, in.num_vars, in.challenge_mode, a.fc, in.st, nullptr,
-                         s_msg, s_r);
+                         s_msg, s_r, false, true);
 }
 
 // Input generation and Montgomery encoding (cuda/src/field_sumcheck.cu:593-611).

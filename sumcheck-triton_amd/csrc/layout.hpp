@@ -65,6 +65,10 @@ struct FieldConsts {
   uint32_t one[MAX_LIMBS];               // R mod p, the Montgomery image of 1
   uint32_t r2[MAX_LIMBS];                // R^2 mod p
   uint32_t word_pow[8][MAX_LIMBS];       // 2^(32i) * R^2 mod p: digest word i -> Montgomery
+  uint32_t small[MAX_POINTS][MAX_LIMBS];  // Montgomery images of 0..7 (interpolation points)
+  // Barycentric weights: bary[n-1][t] = 1 / prod_{s != t, s < n} (t - s), Montgomery, so
+  // g(r) = sum_t g(t) bary[n-1][t] prod_{s != t} (r - s) for a polynomial known at 0..n-1.
+  uint32_t bary[MAX_POINTS][MAX_POINTS][MAX_LIMBS];
 };
 
 // Per-proof device state. polys and r are written by the kernel that finishes a round.

@@ -93,6 +93,40 @@ struct HostField {
       const Elem<L> pw = dbl_times(r2, 32 * w);
       for (int i = 0; i < L; ++i) fc.word_pow[w][i] = pw.v[i];
     }
+    // small[s] = s R mod p; bary[n-1][t] = (-1)^(n-1-t) / (t! (n-1-t)!) in Montgomery form
+    Elem<L> mont_s[MAX_POINTS];
+    mont_s[0] = zero<F>();
+    for (int k = 1; k < MAX_POINTS; ++k) mont_s[k] = add<F>(mont_s[k - 1], rmod);
+    for (int k = 0; k < MAX_POINTS; ++k)
+      for (int i = 0; i < L; ++i) fc.small[k][i] = mont_s[k].v[i];
+    Elem<L> fact[MAX_POINTS];
+    fact[0] = rmod;
+    for (int k = 1; k < MAX_POINTS; ++k) fact[k] = mont_mul<F>(fact[k - 1], mont_s[k]);
+    for (int n = 1; n <= MAX_POINTS; ++n) {
+      for (int t = 0; t < n; ++t) {
+        Elem<L> w = mont_inverse(mont_mul<F>(fact[t], fact[n - 1 - t]), rmod);
+        if ((n - 1 - t) & 1) w = sub<F>(zero<F>(), w);
+        for (int i = 0; i < L; ++i) fc.bary[n - 1][t][i] = w.v[i];
+      }
+    }
+  }
+  // x^(p-2) in the Montgomery domain (the Montgomery image of x^-1 for x != 0).
+  static Elem<L> mont_inverse(const Elem<L>& x, const Elem<L>& one_mont) {
+    Elem<L> e;  // p - 2
+    uint32_t borrow = 2;
+    for (int i = 0; i < L; ++i) {
+      const uint64_t d = (uint64_t)F::p(i) - borrow;
+      e.v[i] = (uint32_t)d;
+      borrow = (uint32_t)(d >> 63);
+    }
+    Elem<L> r = one_mont;
+    for (int i = L - 1; i >= 0; --i) {
+      for (int b = 31; b >= 0; --b) {
+        r = mont_mul<F>(r, r);
+        if ((e.v[i] >> b) & 1u) r = mont_mul<F>(r, x);
+      }
+    }
+    return r;
   }
   static Elem<L> to_mont(const Elem<L>& x, const FieldConsts& fc) {
     return mont_mul<F>(x, from(fc.r2));
