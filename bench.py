@@ -50,11 +50,13 @@ def pass_bytes(T: int, limbs: int, n0: int, j: int) -> int:
 
 
 def modmuls_per_pair(spec, fold: bool) -> int:
-    """Field multiplications per pair of a round: (d+1) evaluations of the composition
-    (each term's len-1 products, plus one for a coefficient other than +-1) and, when the
+    """Field multiplications per pair the kernel performs in a round: the composition at each
+    evaluated point (each term's len-1 products, plus one for a coefficient other than +-1;
+    d+1 points in round 0, d from round 1 on, where g(1) comes from the claim) and, when the
     round folds, two per table."""
     per_point = sum(n - 1 for n in spec.term_lens) + sum(1 for c in spec.coeffs if c not in (1, -1))
-    return (spec.degree + 1) * per_point + (2 * spec.num_vars if fold else 0)
+    points = spec.degree if fold else spec.degree + 1
+    return points * per_point + (2 * spec.num_vars if fold else 0)
 
 
 def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: float | None) -> dict:
