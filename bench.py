@@ -373,7 +373,9 @@ def main() -> int:
                 "mad_u64_u32_lane_ops_per_s": calib["mad_u64_u32"],
                 "hbm_read_gbps": calib["hbm_read"] / 1e9,
                 "hbm_copy_gbps": calib["hbm_copy"] / 1e9,
+                "hbm_read2_write1_gbps": calib["hbm_mix21"] / 1e9,
                 "pass_kernel_frac_of_measured_copy": head["pass_kernel_gbps"] / (calib["hbm_copy"] / 1e9),
+                "pass_kernel_frac_of_measured_read2_write1": head["pass_kernel_gbps"] / (calib["hbm_mix21"] / 1e9),
             },
         },
         "roofline": {

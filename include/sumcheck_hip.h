@@ -199,8 +199,9 @@ int sc_server_destroy(sc_server* server);
 /* Measured sustained lane rates (operations per second) of the VALU operations the field
  * arithmetic is built from: rates[0] v_mad_u64_u32 (32x32+64), rates[1] v_mul_hi_u32,
  * rates[2] 64-bit add, rates[3] v_fma_f64; and streaming HBM bandwidth in bytes/s:
- * rates[4] read-only, rates[5] copy (read + write counted). The INT32 roofline of the wide
- * fields is priced against rates[0]. n <= 6. */
+ * rates[4] read-only, rates[5] copy (read + write counted), rates[6] 2 bytes read per byte
+ * written (a fold round's mix; read + write counted). The INT32 roofline of the wide fields is
+ * priced against rates[0]. n <= 7. */
 int sc_calibrate(sc_context* ctx, double* rates, int32_t n);
 
 #ifdef __cplusplus

@@ -85,6 +85,16 @@ __global__ void __launch_bounds__(256) calib_copy(const uint4* __restrict__ src,
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < words; i += (uint64_t)gridDim.x * 256) dst[i] = src[i];
 }
 
+// Two 16-byte words read per 16-byte word written, contiguous per thread: the traffic shape
+// of a fold round (4 entries read, 2 written, per pair and table).
+__global__ void __launch_bounds__(256) calib_mix21(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                  uint64_t words_out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < words_out; i += (uint64_t)gridDim.x * 256) {
+    const uint4 x = src[2 * i], y = src[2 * i + 1];
+    dst[i] = make_uint4(x.x ^ y.x, x.y ^ y.y, x.z ^ y.z, x.w ^ y.w);
+  }
+}
+
 hipError_t calibrate_bandwidth(hipStream_t s, double* rates, int n) {
   const uint64_t bytes = (uint64_t)1 << 31;  // 2 GiB per buffer: far beyond the 256 MB last-level cache
   const uint64_t words = bytes / 16;
@@ -97,15 +107,17 @@ hipError_t calibrate_bandwidth(hipStream_t s, double* rates, int n) {
   if (e == hipSuccess) e = hipEventCreate(&a);
   if (e == hipSuccess) e = hipEventCreate(&b);
   const int blocks = 4096;
-  for (int k = 0; k < n && k < 2 && e == hipSuccess; ++k) {
+  for (int k = 0; k < n && k < 3 && e == hipSuccess; ++k) {
     float best = 1e30f;
     for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {
       e = hipEventRecord(a, s);
       if (e != hipSuccess) break;
       if (k == 0) {
         hipLaunchKernelGGL(calib_read, dim3(blocks), dim3(256), 0, s, (const uint4*)src, words, (uint32_t*)dst);
-      } else {
+      } else if (k == 1) {
         hipLaunchKernelGGL(calib_copy, dim3(blocks), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, words);
+      } else {
+        hipLaunchKernelGGL(calib_mix21, dim3(blocks), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, words / 2);
       }
       e = hipGetLastError();
       if (e == hipSuccess) e = hipEventRecord(b, s);
@@ -114,8 +126,9 @@ hipError_t calibrate_bandwidth(hipStream_t s, double* rates, int n) {
       if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
       if (e == hipSuccess && rep > 0 && ms < best) best = ms;
     }
-    // bytes moved per second: read probe reads `bytes`, copy reads and writes them
-    if (e == hipSuccess) rates[k] = (k == 0 ? 1.0 : 2.0) * (double)bytes / (best * 1e-3);
+    // bytes moved per second: read probe reads `bytes`, copy reads and writes them, the 2:1
+    // probe reads them and writes half as many
+    if (e == hipSuccess) rates[k] = (k == 0 ? 1.0 : (k == 1 ? 2.0 : 1.5)) * (double)bytes / (best * 1e-3);
   }
   if (a) (void)hipEventDestroy(a);
   if (b) (void)hipEventDestroy(b);

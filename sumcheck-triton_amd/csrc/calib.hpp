@@ -12,11 +12,12 @@ enum CalibOp {
   CALIB_FMA_F64 = 3,
   CALIB_HBM_READ = 4,  // bytes/s, streaming read of 2 GiB
   CALIB_HBM_COPY = 5,  // bytes/s read + written, streaming copy of 2 GiB
-  CALIB_COUNT = 6
+  CALIB_HBM_MIX21 = 6, // bytes/s read + written, 2 GiB read while 1 GiB is written (a fold's mix)
+  CALIB_COUNT = 7
 };
 
 // rates[k] = sustained lane operations per second of op k (k < 4) or HBM bytes per second
-// (k = 4, 5); n <= CALIB_COUNT entries.
+// (k = 4, 5, 6); n <= CALIB_COUNT entries.
 hipError_t calibrate(hipStream_t s, double* rates, int n);
 
 }  // namespace sc

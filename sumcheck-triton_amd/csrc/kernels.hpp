@@ -517,7 +517,7 @@ __global__ void __launch_bounds__(BLOCK) finalize_kernel(FinalizeArgs a) {
 // One workgroup of TB threads (more than a pass block: the tail's rounds have no other
 // workgroups to overlap with, so it uses a whole CU).
 template <class F, int TMAX, int PMAX, int SID, int TB>
-__global__ void __launch_bounds__(TB) tail_kernel(TailArgs a) {
+__device__ __forceinline__ void tail_rounds(const TailArgs& a) {
   constexpr int L = F::L;
   __shared__ uint32_t s_red[(TB / 64) * PMAX * L];
   __shared__ uint32_t s_g[PMAX * L];
@@ -579,6 +579,11 @@ __global__ void __launch_bounds__(TB) tail_kernel(TailArgs a) {
   }
   __syncthreads();
   if (threadIdx.x == 0) a.st->final_ready = 1u;
+}
+
+template <class F, int TMAX, int PMAX, int SID, int TB>
+__global__ void __launch_bounds__(TB) tail_kernel(TailArgs a) {
+  tail_rounds<F, TMAX, PMAX, SID, TB>(a);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -727,6 +732,28 @@ __global__ void __launch_bounds__(BLOCK) batch_pass_kernel(BatchArgs a) {
   if (threadIdx.x < npts * L) {
     a.partials[(uint64_t)blockIdx.x * (MAX_POINTS * MAX_LIMBS) + threadIdx.x] = s_g[threadIdx.x];
   }
+}
+
+// One block per entry: the entry's instance runs all its remaining rounds from entry.round in
+// this block, exactly as tail_kernel does for a single proof (its tables are small by then).
+template <class F, int TMAX, int PMAX>
+__global__ void __launch_bounds__(BLOCK) batch_tail_kernel(BatchArgs a) {
+  const StageEntry ent = a.entries[blockIdx.x];
+  const InstDesc& in = a.insts[ent.inst];
+  TailArgs t;
+  t.input = in.input;
+  t.buf_a = in.buf_a;
+  t.buf_b = in.buf_b;
+  t.n0 = in.n0;
+  t.first_round = (int32_t)ent.round;
+  t.num_rounds = in.num_vars;
+  t.challenge_mode = in.challenge_mode;
+  t.want_final = 0;
+  t.comp = in.comp;
+  t.fc = a.fc;
+  t.st = in.st;
+  t.explicit_r = nullptr;
+  tail_rounds<F, TMAX, PMAX, 0, BLOCK>(t);
 }
 
 // One block per entry: sum the entry's tile partials, store g_round, derive r_round.

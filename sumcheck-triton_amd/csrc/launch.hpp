@@ -22,6 +22,8 @@ struct Launchers {
   // batched stages: grid = tiles of the stage / entries of the stage
   hipError_t (*batch_pass)(hipStream_t, int tmax, int pmax, const BatchArgs&, unsigned tiles);
   hipError_t (*batch_finalize)(hipStream_t, int pmax, const BatchArgs&, unsigned entries);
+  // one block per entry running all of its instance's remaining rounds
+  hipError_t (*batch_tail)(hipStream_t, int tmax, int pmax, const BatchArgs&, unsigned entries);
   // wide interpreted compositions (tmax > 8): fold launch (round >= 1), then a point-parallel
   // evaluation launch with pmax lanes per pair
   hipError_t (*wide_fold)(hipStream_t, const PassArgs&, unsigned grid);

@@ -849,7 +849,7 @@ int sc_transcript_challenge(sc_context* ctx, int32_t bit_width, int32_t round, c
 
 int sc_calibrate(sc_context* ctx, double* rates, int32_t n) {
   if (ctx == nullptr || rates == nullptr) return fail(SC_ERR_ARG, "NULL argument");
-  if (n < 0 || n > CALIB_COUNT) return fail(SC_ERR_ARG, "n must be in [0, 6]");
+  if (n < 0 || n > CALIB_COUNT) return fail(SC_ERR_ARG, "n must be in [0, 7]");
   SC_HIP(hipSetDevice(ctx->device));
   for (int k = 0; k < n; ++k) rates[k] = 0.0;
   SC_HIP(calibrate(ctx->stream, rates, n));
@@ -886,6 +886,13 @@ struct sc_batch {
   StageEntry* d_entries = nullptr;
   uint32_t* d_partials = nullptr;
   std::vector<uint32_t> stage_offset, stage_entries, stage_tiles;
+  // Each instance's last rounds (from tail_round[i] on, at most tail_pairs pairs) run as one
+  // tail entry of the last stage: one workgroup loops over them, as a single proof's tail
+  // kernel does. All tails share that one launch (a tail launch per stage measured slower:
+  // each adds the latency of its slowest instance's several rounds to the critical path).
+  std::vector<uint32_t> tail_offset, tail_entries;
+  std::vector<int> tail_round;
+  int num_stages = 0;
   uint32_t items = 4;
   int mode = -1;  // challenge mode baked into the instance descriptors
   std::vector<InstDesc> insts;
@@ -920,16 +927,22 @@ void drop_batch_graph(sc_batch* b) {
 // Work lists of every stage for the current tile size, uploaded to the device.
 int build_stages(sc_batch* b) {
   std::vector<StageEntry> all;
-  b->stage_offset.assign(b->max_rounds, 0);
-  b->stage_entries.assign(b->max_rounds, 0);
-  b->stage_tiles.assign(b->max_rounds, 0);
+  b->num_stages = 0;
+  for (int i = 0; i < b->n; ++i) b->num_stages = std::max(b->num_stages, b->tail_round[i] + 1);
+  const int last = b->num_stages - 1;
+  const int ns = b->num_stages;
+  b->stage_offset.assign(ns, 0);
+  b->stage_entries.assign(ns, 0);
+  b->stage_tiles.assign(ns, 0);
+  b->tail_offset.assign(ns, 0);
+  b->tail_entries.assign(ns, 0);
   uint64_t max_tiles = 1;
   const uint64_t tile_pairs = (uint64_t)BLOCK * b->items;
-  for (int s = 0; s < b->max_rounds; ++s) {
+  for (int s = 0; s < ns; ++s) {
     b->stage_offset[s] = (uint32_t)all.size();
     uint64_t tiles = 0;
     for (int i = 0; i < b->n; ++i) {
-      if (s >= b->num_vars[i]) continue;
+      if (s >= b->tail_round[i]) continue;
       const uint64_t pairs = b->n0[i] >> (s + 1);
       const uint64_t t = (pairs + tile_pairs - 1) / tile_pairs;
       if (tiles + t > 0x7FFFFFFFull) return fail(SC_ERR_UNSUPPORTED, "batch stage has too many tiles");
@@ -939,6 +952,11 @@ int build_stages(sc_batch* b) {
     b->stage_entries[s] = (uint32_t)all.size() - b->stage_offset[s];
     b->stage_tiles[s] = (uint32_t)tiles;
     if (tiles > max_tiles) max_tiles = tiles;
+    b->tail_offset[s] = (uint32_t)all.size();
+    if (s == last) {
+      for (int i = 0; i < b->n; ++i) all.push_back(StageEntry{(uint32_t)i, (uint32_t)b->tail_round[i], 0u, 0u});
+    }
+    b->tail_entries[s] = (uint32_t)all.size() - b->tail_offset[s];
   }
   if (b->d_entries) (void)hipFree(b->d_entries);
   if (b->d_partials) (void)hipFree(b->d_partials);
@@ -965,17 +983,23 @@ int set_batch_mode(sc_batch* b, int mode) {
 
 int enqueue_batch(sc_batch* b) {
   hipStream_t s = b->ctx->stream;
-  for (int st = 0; st < b->max_rounds; ++st) {
-    if (b->stage_entries[st] == 0) continue;
+  for (int st = 0; st < b->num_stages; ++st) {
     BatchArgs a;
     a.insts = b->d_insts;
-    a.entries = b->d_entries + b->stage_offset[st];
-    a.n_entries = b->stage_entries[st];
     a.items = b->items;
     a.partials = b->d_partials;
     a.fc = b->ctx->d_fc[b->fidx];
-    SC_HIP(b->ln->batch_pass(s, b->tmax, b->pmax, a, b->stage_tiles[st]));
-    SC_HIP(b->ln->batch_finalize(s, b->pmax, a, b->stage_entries[st]));
+    if (b->stage_entries[st] != 0) {
+      a.entries = b->d_entries + b->stage_offset[st];
+      a.n_entries = b->stage_entries[st];
+      SC_HIP(b->ln->batch_pass(s, b->tmax, b->pmax, a, b->stage_tiles[st]));
+      SC_HIP(b->ln->batch_finalize(s, b->pmax, a, b->stage_entries[st]));
+    }
+    if (b->tail_entries[st] != 0) {
+      a.entries = b->d_entries + b->tail_offset[st];
+      a.n_entries = b->tail_entries[st];
+      SC_HIP(b->ln->batch_tail(s, b->tmax, b->pmax, a, b->tail_entries[st]));
+    }
   }
   return SC_OK;
 }
@@ -1003,6 +1027,7 @@ int sc_batch_create(sc_context* ctx, int32_t bit_width, int32_t n, const sc_comp
   b->num_vars.resize(n);
   b->npts.resize(n);
   b->n0.resize(n);
+  b->tail_round.resize(n);
   int maxT = 0, maxP = 0;
   std::vector<uint64_t> offs(n);
   uint64_t total = 0;
@@ -1034,6 +1059,12 @@ int sc_batch_create(sc_context* ctx, int32_t bit_width, int32_t n, const sc_comp
   }
   b->tmax = pick_tmax(maxT);
   b->pmax = maxP <= 4 ? 4 : 8;
+  for (int i = 0; i < n; ++i) {  // first round whose pairs fit the tail (as for a single proof)
+    const int64_t tp = default_tail_pairs(b->L, b->tmax);
+    int j = 0;
+    while (j < b->num_vars[i] - 1 && (int64_t)(b->n0[i] >> (j + 1)) > tp) ++j;
+    b->tail_round[i] = j;
+  }
   SC_HIP_B(hipSetDevice(ctx->device));
   SC_HIP_B(hipMalloc(&b->pool, total));
   b->pool_bytes = total;
@@ -1169,7 +1200,7 @@ int sc_batch_result(sc_batch* b, int32_t i, uint32_t* round_polys, uint32_t* cha
 int sc_batch_info(sc_batch* b, uint64_t* device_bytes, int32_t* stages, uint64_t* tiles) {
   if (b == nullptr) return fail(SC_ERR_ARG, "batch must not be NULL");
   if (device_bytes) *device_bytes = b->pool_bytes;
-  if (stages) *stages = b->max_rounds;
+  if (stages) *stages = b->num_stages;
   if (tiles) {
     uint64_t t = 0;
     for (uint32_t x : b->stage_tiles) t += x;
@@ -1509,7 +1540,9 @@ int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
   int staged = 0;
   bool any = false;
   for (int fidx = 0; fidx < 4; ++fidx) {
-    std::vector<StageEntry> ents;
+    // pass entries (one round each) first, then tail entries: requests whose current round
+    // has at most the tail's pairs finish all their remaining rounds in this step
+    std::vector<StageEntry> ents, tails;
     int maxT = 0, maxP = 0;
     uint64_t tiles = 0;
     const uint64_t tile_pairs = (uint64_t)BLOCK * sv->items;
@@ -1519,14 +1552,21 @@ int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
       Request& r = sv->reqs[id];
       if (r.state != REQ_RUNNING || r.fidx != fidx) continue;
       const uint64_t pairs = ((uint64_t)1 << r.num_vars) >> (r.round + 1);
-      const uint64_t t = (pairs + tile_pairs - 1) / tile_pairs;
-      if (tiles + t > sv->max_tiles) break;  // the rest run next step
-      ents.push_back(StageEntry{(uint32_t)slot, (uint32_t)r.round, (uint32_t)tiles, (uint32_t)t});
-      tiles += t;
+      const int64_t tail_pairs = default_tail_pairs(r.L, pick_tmax(r.desc.num_tables));
+      if ((int64_t)pairs <= tail_pairs) {
+        tails.push_back(StageEntry{(uint32_t)slot, (uint32_t)r.round, 0u, 0u});
+      } else {
+        const uint64_t t = (pairs + tile_pairs - 1) / tile_pairs;
+        if (tiles + t > sv->max_tiles) continue;  // runs in a later step
+        ents.push_back(StageEntry{(uint32_t)slot, (uint32_t)r.round, (uint32_t)tiles, (uint32_t)t});
+        tiles += t;
+      }
       maxT = std::max(maxT, r.desc.num_tables);
       maxP = std::max(maxP, r.npts);
     }
-    if (ents.empty()) continue;
+    if (ents.empty() && tails.empty()) continue;
+    const uint32_t n_pass = (uint32_t)ents.size();
+    ents.insert(ents.end(), tails.begin(), tails.end());
     any = true;
     const int si = take_staging(sv);
     if (si < 0) return fail(SC_ERR_HIP, "no staging buffer");
@@ -1538,18 +1578,26 @@ int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
     ++staged;
     BatchArgs a;
     a.insts = sv->d_insts;
-    a.entries = sv->d_entries;
-    a.n_entries = (uint32_t)ents.size();
     a.items = sv->items;
     a.partials = sv->d_partials;
     a.fc = ctx->d_fc[fidx];
     const Launchers& ln = launchers_for(fidx);
     const int tmax = pick_tmax(maxT), pmax = maxP <= 4 ? 4 : 8;
-    SC_HIP(ln.batch_pass(s, tmax, pmax, a, (unsigned)tiles));
-    SC_HIP(ln.batch_finalize(s, pmax, a, (unsigned)ents.size()));
-    for (const StageEntry& e : ents) {
+    if (n_pass) {
+      a.entries = sv->d_entries;
+      a.n_entries = n_pass;
+      SC_HIP(ln.batch_pass(s, tmax, pmax, a, (unsigned)tiles));
+      SC_HIP(ln.batch_finalize(s, pmax, a, n_pass));
+    }
+    if (ents.size() > n_pass) {
+      a.entries = sv->d_entries + n_pass;
+      a.n_entries = (uint32_t)ents.size() - n_pass;
+      SC_HIP(ln.batch_tail(s, tmax, pmax, a, a.n_entries));
+    }
+    for (size_t k = 0; k < ents.size(); ++k) {
+      const StageEntry& e = ents[k];
       Request& r = sv->reqs[sv->slot_req[e.inst]];
-      r.round += 1;
+      r.round = k < n_pass ? r.round + 1 : r.num_vars;
       if (r.round >= r.num_vars) {
         SC_HIP(hipMemcpyAsync(r.result, sv->d_states + r.slot, sizeof(ProofState), hipMemcpyDeviceToHost, s));
         r.state = REQ_FINISHING;

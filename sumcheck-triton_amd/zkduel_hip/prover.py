@@ -130,10 +130,10 @@ class Context:
 
     def calibrate(self) -> dict[str, float]:
         """Measured lane rates (ops/s) of mad_u64_u32, mul_hi_u32, 64-bit add and fma_f64, and
-        streaming HBM read / copy bandwidth (bytes/s)."""
-        rates = (ctypes.c_double * 6)()
-        _lib.check(self.lib.sc_calibrate(self.handle, rates, 6))
-        keys = ("mad_u64_u32", "mul_hi_u32", "add64", "fma_f64", "hbm_read", "hbm_copy")
+        streaming HBM bandwidth (bytes/s): read-only, copy, and 2 bytes read per byte written."""
+        rates = (ctypes.c_double * 7)()
+        _lib.check(self.lib.sc_calibrate(self.handle, rates, 7))
+        keys = ("mad_u64_u32", "mul_hi_u32", "add64", "fma_f64", "hbm_read", "hbm_copy", "hbm_mix21")
         return dict(zip(keys, list(rates)))
 
     def transcript_challenge(self, bit_width: int, round_idx: int, g_mont: np.ndarray) -> np.ndarray:
