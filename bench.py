@@ -67,7 +67,10 @@ def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: floa
     spec = get_spec(WORKLOAD)
     comp = Composition.from_spec(spec)
     T = spec.num_vars
+    from zkduel_hip.autotune import tune_proof
+
     with Proof(ctx, bw, comp, nv, input="uniform", seed=SEED) as pr:
+        schedule = tune_proof(pr)  # grid cap and tail depth timed on this shape, then kept
         pr.set_profiling(True)
         for _ in range(warmup):
             pr.run("sha3")
@@ -112,6 +115,7 @@ def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: floa
         "hbm_frac": achieved / HBM_PEAK_GBPS,
         "pass_limb_macs_per_s": pass_macs_sum / (pass_ms_sum * 1e-3) if pass_ms_sum > 0 else 0.0,
         "int32_frac": (pass_macs_sum / (pass_ms_sum * 1e-3)) / mad_rate if (mad_rate and pass_ms_sum > 0) else None,
+        "schedule": {"max_pass_blocks": schedule["max_pass_blocks"], "tail_pairs": schedule["tail_pairs"]},
         "wall_s": wall,
     }
 

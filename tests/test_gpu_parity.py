@@ -370,3 +370,22 @@ def test_full_size_protocol_invariants(ctx, bw, nv):
         assert r[j] == R.transcript_challenge(j, g[j], p, f.limbs)
     for j in range(1, nv):
         assert (g[j][0] + g[j][1]) % p == R.lagrange_eval(g[j - 1], r[j - 1], p), j
+
+
+def test_autotuned_schedule_keeps_results(ctx):
+    """The autotuner only changes launch schedules: the tuned proof gives the same bits."""
+    from zkduel_hip import autotune
+
+    autotune._cache.clear()
+    comp = Composition.from_spec(get_spec("poly_abc_plus_de"))
+    with Proof(ctx, 64, comp, 16, input="uniform", seed=8) as pr:
+        pr.run("sha3")
+        pr.wait()
+        want = pr.result()
+        choice = autotune.tune_proof(pr, repeats=1)
+        say("tuned", choice["max_pass_blocks"], choice["tail_pairs"])
+        pr.run("sha3")
+        pr.wait()
+        got = pr.result()
+    assert np.array_equal(got[0], want[0]) and got[2] == want[2]
+    autotune._cache.clear()

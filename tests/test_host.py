@@ -160,3 +160,20 @@ def test_driver_columns_are_the_reference_csv_header():
               "rounds,N,vars,degree,terms,warmups,repeats,timing_mode,challenge_mode,first_ms,median_ms,"
               "eval_median_ms,challenge_median_ms,fold_median_ms,total_median_ms,input_mib,checksum,status")
     assert ",".join(CSV_COLUMNS) == header  # cuda/src/field_sumcheck.cu:1336
+
+
+def test_autotuner_cache_roundtrip(tmp_path):
+    """The schedule cache file round-trips (CPU only: no proof is run)."""
+    import json
+
+    from zkduel_hip import autotune
+
+    path = tmp_path / "tune.json"
+    autotune._cache.clear()
+    autotune._cache["64/5/3/26"] = {"max_pass_blocks": 1024, "tail_pairs": 512, "ms": 2.7, "candidates": {}}
+    autotune._store(path)
+    autotune._cache.clear()
+    autotune._load(path)
+    assert autotune._cache["64/5/3/26"]["max_pass_blocks"] == 1024
+    assert json.loads(path.read_text())["64/5/3/26"]["tail_pairs"] == 512
+    autotune._cache.clear()
