@@ -59,7 +59,8 @@ def modmuls_per_pair(spec, fold: bool) -> int:
     return points * per_point + (2 * spec.num_vars if fold else 0)
 
 
-def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: float | None) -> dict:
+def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: float | None,
+                  fixed_schedule: tuple[int, int] | None = None) -> dict:
     from zkduel_hip import Composition, Proof, get_field, get_spec
     from zkduel_hip.driver import algorithmic_bytes
 
@@ -70,7 +71,11 @@ def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: floa
     from zkduel_hip.autotune import tune_proof
 
     with Proof(ctx, bw, comp, nv, input="uniform", seed=SEED) as pr:
-        schedule = tune_proof(pr)  # grid cap and tail depth timed on this shape, then kept
+        if fixed_schedule:
+            pr.set_schedule(fixed_schedule[0], fixed_schedule[1], False)
+            schedule = {"max_pass_blocks": fixed_schedule[0], "tail_pairs": fixed_schedule[1]}
+        else:
+            schedule = tune_proof(pr)  # grid cap and tail depth timed on this shape, then kept
         pr.set_profiling(True)
         for _ in range(warmup):
             pr.run("sha3")
@@ -278,13 +283,18 @@ def cpu_baseline(budget_s: float = 20.0) -> dict:
     }
 
 
-def traffic_bytes():
+def traffic_bytes(pass_rounds: int):
     """HBM bytes per pass-kernel launch measured by rocprofv3 FETCH_SIZE + WRITE_SIZE (committed
-    under profiles/ with the command that produced it), or None."""
+    under profiles/ with the command that produced it), averaged over the pass rounds this run
+    used (rounds 0..pass_rounds-1), or None."""
     path = ROOT / "profiles" / "pass_kernel_traffic_goldilocks_nv26.json"
     if not path.exists():
         return None
-    return json.loads(path.read_text()).get("traffic_bytes_per_launch")
+    d = json.loads(path.read_text())
+    per = d.get("traffic_bytes_per_round")
+    if per and 0 < pass_rounds <= len(per):
+        return int(round(sum(per[:pass_rounds]) / pass_rounds))
+    return d.get("traffic_bytes_per_launch")
 
 
 def main() -> int:
@@ -295,6 +305,9 @@ def main() -> int:
     ap.add_argument("--widths", type=int, nargs="+", default=[64, 32, 128, 256])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--pass-blocks", type=int, default=0,
+                    help="fixed pass-grid cap for the per-width proofs (with --tail-pairs: skip the autotuner)")
+    ap.add_argument("--tail-pairs", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -312,7 +325,8 @@ def main() -> int:
     for bw in args.widths:
         steps = args.steps if bw == 64 else max(3, args.steps // 2)
         log(f"bench: width {bw} num_vars {SIZES[bw]} steps {steps}")
-        per[bw] = measure_width(ctx, bw, SIZES[bw], steps, args.warmup, calib["mad_u64_u32"])
+        fixed = (args.pass_blocks, args.tail_pairs) if (args.pass_blocks and args.tail_pairs) else None
+        per[bw] = measure_width(ctx, bw, SIZES[bw], steps, args.warmup, calib["mad_u64_u32"], fixed)
         log(f"bench: width {bw}: {per[bw]['proofs_per_s']:.1f} proofs/s, "
             f"pass kernel {per[bw]['pass_kernel_gbps']:.0f} GB/s")
     head = per[64]
@@ -388,7 +402,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(head["pass_kernel_gbps"] / HBM_PEAK_GBPS, 4),
-            "traffic": traffic_bytes(),
+            "traffic": traffic_bytes(head["pass_rounds"]),
         },
         "cpu_baseline": cpu,
     }
