@@ -1,6 +1,6 @@
 """Summarize a rocprofv3 results database (``--kernel-trace`` and/or ``--pmc``) as text.
 
-    python -m zkduel_hip.rocprof_summary <results.db> [--pmc COUNTER] [--match SUBSTRING]
+    python -m zkduel_hip.rocprof_summary <results.db> [--pmc COUNTER] [--match SUBSTRING] [--dispatches]
 
 Prints per-kernel launch counts and durations (avg/min/max/total), and, with --pmc, the
 counter's average value per dispatch of each kernel. Counters such as FETCH_SIZE and
@@ -39,16 +39,32 @@ def pmc_per_kernel(db: sqlite3.Connection, counter: str) -> dict[str, tuple[int,
     return {k: (len(v), sum(v) / len(v)) for k, v in acc.items()}
 
 
+def pmc_per_dispatch(db: sqlite3.Connection, counter: str, match: str) -> list[tuple[int, float]]:
+    """(dispatch id, counter value summed over the dispatch's records) of every dispatch of the
+    kernels whose name contains ``match``, in dispatch order."""
+    return [(int(d), float(v)) for d, v in db.execute(
+        "select dispatch_id, sum(counter_value) from pmc_events where counter_name = ? and name like ? "
+        "group by dispatch_id order by dispatch_id", (counter, f"%{match}%"))]
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("db")
     ap.add_argument("--pmc", action="append", default=[])
     ap.add_argument("--match", default=None)
+    ap.add_argument("--dispatches", action="store_true",
+                    help="with --pmc and --match: print the counter of every matching dispatch")
     args = ap.parse_args()
     db = sqlite3.connect(args.db)
     print(f"{'kernel':70s} {'n':>5s} {'avg_us':>10s} {'min_us':>10s} {'max_us':>10s} {'total_ms':>10s}")
     for name, n, avg, mn, mx, tot in kernel_stats(db, args.match):
         print(f"{name[:70]:70s} {n:5d} {avg / 1e3:10.2f} {mn / 1e3:10.2f} {mx / 1e3:10.2f} {tot / 1e6:10.3f}")
+    if args.dispatches and args.match:
+        for counter in args.pmc:
+            print(f"\n{counter} of each dispatch of kernels matching {args.match!r}")
+            for d, v in pmc_per_dispatch(db, counter, args.match):
+                print(f"{d:8d} {v:16.1f}")
+        return 0
     for counter in args.pmc:
         print(f"\n{counter} per dispatch (average over dispatches)")
         for name, (n, avg) in sorted(pmc_per_kernel(db, counter).items(), key=lambda kv: -kv[1][1]):
