@@ -129,8 +129,8 @@ int sc_proof_use_compiled(sc_proof* proof, int32_t enable);
 int sc_proof_set_profiling(sc_proof* proof, int32_t enable);
 int sc_proof_kernel_times(sc_proof* proof, float* pass_ms, int32_t capacity, float* tail_ms);
 
-/* Final-point step of the last run (survey 8(f) item 4; the reference stops after the last
- * round polynomial, field_sumcheck.cu:1057): r_final [L] = the challenge derived from the last
+/* Final-point step of the last run (no counterpart in the reference, which stops after the
+ * last round polynomial, field_sumcheck.cu:1057): r_final [L] = the challenge derived from the last
  * round's polynomial by the same transcript rule, final_values [T][L] = every table folded at
  * (r_0, ..., r_final), i.e. f_v at the random point. Montgomery limbs. A verifier checks
  * g_last(r_final) = C(final_values). SC_ERR_UNSUPPORTED after a run with explicit challenges. */
