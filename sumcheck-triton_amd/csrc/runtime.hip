@@ -254,6 +254,7 @@ struct sc_proof {
   ProofState* d_state = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int max_pass_blocks = 2048;
+  bool default_blocks = true;  // grid cap follows the kernel choice (no caller-set value)
   int64_t tail_pairs = 0;
   bool fused_finalize = false;
   hipGraph_t graph = nullptr;
@@ -268,14 +269,12 @@ struct sc_proof {
 
 namespace {
 
-// Schedule defaults per field, from sweeps on MI355X (a*b*c + d*e at num_vars 24-26): the
-// pass grid cap, and the pair count at which the remaining rounds move to the tail kernel.
 // Pass-kernel grid caps from sweeps on MI355X (a*b*c + d*e at the benchmark sizes; fewer
 // blocks also means fewer partials for the finalize kernel to sum). Compositions of more than
-// 8 tables use the point-parallel kernel, which holds few registers per lane and wants the
-// largest grid (BLS12-381 constraint workload: 22.9 ms at 512 blocks, 19.4 ms at 4096).
-int default_pass_blocks(int L, int tmax) {
-  if (tmax > 8) return MAX_PASS_BLOCKS;
+// 8 tables (and interpreted 255-bit compositions) use the point-parallel kernel, which holds
+// few registers per lane and wants the largest grid (BLS12-381 constraint workload: 22.9 ms at 512 blocks, 19.4 ms at 4096).
+int default_pass_blocks(int L, bool wide) {
+  if (wide) return MAX_PASS_BLOCKS;
   switch (L) {
     case 1: return 2048;
     case 2: return 1024;
@@ -287,10 +286,15 @@ int default_pass_blocks(int L, int tmax) {
 // Pairs below which the remaining rounds run in the single-workgroup tail kernel. Compositions
 // of more than 8 tables skip it: their per-pair work is large enough that one workgroup is
 // slower than the fold + point-parallel launches even for a handful of pairs.
-int64_t default_tail_pairs(int L, int tmax) {
-  if (tmax > 8) return 1;
+int64_t default_tail_pairs(int L, bool over8) {
+  if (over8) return 1;
   return L == 8 ? 64 : 512;
 }
+
+// Interpreted compositions of more than 8 tables, and every interpreted composition of the
+// 255-bit field (whose per-pair kernels need one wave per SIMD for 2 x 8 working values and 8
+// accumulators of 9 limbs), run as a fold launch plus the point-parallel evaluation.
+bool proof_is_wide(const sc_proof* p) { return p->sid == 0 && (p->tmax > 8 || p->L == 8); }
 
 // First round that runs in the tail kernel.
 int tail_first_round(const sc_proof* p) {
@@ -328,7 +332,7 @@ int enqueue_rounds(sc_proof* p, int mode) {
     uint64_t g = (a.pairs + BLOCK - 1) / BLOCK;
     if (g > (uint64_t)p->max_pass_blocks) g = (uint64_t)p->max_pass_blocks;
     // Interpreted compositions of more than 8 tables: fold launch + point-parallel evaluation.
-    const bool wide = p->sid == 0 && p->tmax > 8;
+    const bool wide = proof_is_wide(p);
     if (p->profiling) SC_HIP(hipEventRecord(p->prof_ev[2 * j], s));
     if (wide) {
       if (j > 0) {
@@ -510,8 +514,8 @@ int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* co
   p->tmax = pick_tmax(p->T);
   p->pmax = p->npts <= 4 ? 4 : 8;
   p->n0 = (uint64_t)1 << num_vars;
-  p->tail_pairs = default_tail_pairs(p->L, p->tmax);
-  p->max_pass_blocks = default_pass_blocks(p->L, p->tmax);
+  p->tail_pairs = default_tail_pairs(p->L, p->tmax > 8);
+  p->max_pass_blocks = default_pass_blocks(p->L, proof_is_wide(p));
   p->matched_sid = match_static(desc);
   p->sid = p->matched_sid;
 
@@ -597,8 +601,9 @@ int sc_proof_set_schedule(sc_proof* p, int32_t max_pass_blocks, int64_t tail_pai
   if (max_pass_blocks < 0 || max_pass_blocks > MAX_PASS_BLOCKS)
     return fail(SC_ERR_ARG, "max_pass_blocks must be in [0, 4096]");
   if (tail_pairs < 0) return fail(SC_ERR_ARG, "tail_pairs must be >= 0");
-  p->max_pass_blocks = max_pass_blocks ? max_pass_blocks : default_pass_blocks(p->L, p->tmax);
-  p->tail_pairs = tail_pairs ? tail_pairs : default_tail_pairs(p->L, p->tmax);
+  p->default_blocks = max_pass_blocks == 0;
+  p->max_pass_blocks = max_pass_blocks ? max_pass_blocks : default_pass_blocks(p->L, proof_is_wide(p));
+  p->tail_pairs = tail_pairs ? tail_pairs : default_tail_pairs(p->L, p->tmax > 8);
   if (p->graph_exec) {
     (void)hipGraphExecDestroy(p->graph_exec);
     (void)hipGraphDestroy(p->graph);
@@ -745,6 +750,7 @@ int sc_proof_use_compiled(sc_proof* p, int32_t enable) {
     p->graph_mode = -1;
   }
   p->sid = sid;
+  if (p->default_blocks) p->max_pass_blocks = default_pass_blocks(p->L, proof_is_wide(p));
   return p->matched_sid;
 }
 
@@ -1060,7 +1066,7 @@ int sc_batch_create(sc_context* ctx, int32_t bit_width, int32_t n, const sc_comp
   b->tmax = pick_tmax(maxT);
   b->pmax = maxP <= 4 ? 4 : 8;
   for (int i = 0; i < n; ++i) {  // first round whose pairs fit the tail (as for a single proof)
-    const int64_t tp = default_tail_pairs(b->L, b->tmax);
+    const int64_t tp = default_tail_pairs(b->L, b->tmax > 8);
     int j = 0;
     while (j < b->num_vars[i] - 1 && (int64_t)(b->n0[i] >> (j + 1)) > tp) ++j;
     b->tail_round[i] = j;
@@ -1552,7 +1558,7 @@ int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
       Request& r = sv->reqs[id];
       if (r.state != REQ_RUNNING || r.fidx != fidx) continue;
       const uint64_t pairs = ((uint64_t)1 << r.num_vars) >> (r.round + 1);
-      const int64_t tail_pairs = default_tail_pairs(r.L, pick_tmax(r.desc.num_tables));
+      const int64_t tail_pairs = default_tail_pairs(r.L, pick_tmax(r.desc.num_tables) > 8);
       if ((int64_t)pairs <= tail_pairs) {
         tails.push_back(StageEntry{(uint32_t)slot, (uint32_t)r.round, 0u, 0u});
       } else {
