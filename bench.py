@@ -75,7 +75,9 @@ def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: floa
             pr.set_schedule(fixed_schedule[0], fixed_schedule[1], False)
             schedule = {"max_pass_blocks": fixed_schedule[0], "tail_pairs": fixed_schedule[1]}
         else:
-            schedule = tune_proof(pr)  # grid cap and tail depth timed on this shape, then kept
+            schedule = tune_proof(pr)  # kernel choice, grid cap and tail depth timed on this shape
+        # rounds at or below this pair count run fold + point-parallel evaluation, not pass_kernel
+        wide_pairs = pr.info()["point_parallel_pairs"]
         pr.set_profiling(True)
         for _ in range(warmup):
             pr.run("sha3")
@@ -93,6 +95,8 @@ def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: floa
             lat.append((time.perf_counter() - s0) * 1000.0)
             passes, _tail = pr.kernel_times()
             for j, ms in enumerate(passes):
+                if ((1 << nv) >> (j + 1)) <= wide_pairs:
+                    continue
                 pass_ms_sum += ms
                 pass_bytes_sum += pass_bytes(T, f.limbs, 1 << nv, j)
                 pass_macs_sum += ((1 << nv) >> (j + 1)) * modmuls_per_pair(spec, j > 0) * limb_macs
@@ -117,10 +121,12 @@ def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: floa
         "pass_kernel_avg_ms": pass_ms_sum / max(pass_launches, 1),
         "pass_kernel_avg_bytes": pass_bytes_sum / max(pass_launches, 1),
         "pass_rounds": info["pass_rounds"],
+        "pass_kernel_rounds": sum(1 for j in range(info["pass_rounds"]) if ((1 << nv) >> (j + 1)) > wide_pairs),
         "hbm_frac": achieved / HBM_PEAK_GBPS,
         "pass_limb_macs_per_s": pass_macs_sum / (pass_ms_sum * 1e-3) if pass_ms_sum > 0 else 0.0,
         "int32_frac": (pass_macs_sum / (pass_ms_sum * 1e-3)) / mad_rate if (mad_rate and pass_ms_sum > 0) else None,
-        "schedule": {"max_pass_blocks": schedule["max_pass_blocks"], "tail_pairs": schedule["tail_pairs"]},
+        "schedule": {"max_pass_blocks": schedule["max_pass_blocks"], "tail_pairs": schedule["tail_pairs"],
+                     "point_parallel_pairs": wide_pairs},
         "wall_s": wall,
     }
 
@@ -402,7 +408,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(head["pass_kernel_gbps"] / HBM_PEAK_GBPS, 4),
-            "traffic": traffic_bytes(head["pass_rounds"]),
+            "traffic": traffic_bytes(head["pass_kernel_rounds"]),
         },
         "cpu_baseline": cpu,
     }

@@ -93,6 +93,15 @@ int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* co
 int sc_proof_set_schedule(sc_proof* proof, int32_t max_pass_blocks, int64_t tail_pairs,
                           int32_t fused_finalize);
 
+/* Kernel choice per round: rounds with at most max_pairs pairs run as a fold launch plus the
+ * point-parallel evaluation (one lane per pair and point) instead of the per-pair pass kernel
+ * with the fold fused in; 0 never, a negative value restores the default (every round for
+ * interpreted compositions of more than 8 tables, rounds below 2^20 pairs for interpreted
+ * 255-bit compositions, none otherwise). Results do not depend on it. No counterpart in the
+ * reference, which runs every round as an eval, a reduce and a separate fold kernel
+ * (field_sumcheck.cu:614-694). */
+int sc_proof_set_point_parallel(sc_proof* proof, int64_t max_pairs);
+
 /* Enqueues one whole proof (all rounds, transcript on the device) on the context's stream
  * and returns without waiting: the reference's run_once (field_sumcheck.cu:1012-1088).
  * explicit_r: [num_vars-1][L] canonical, only for SC_CHALLENGE_EXPLICIT (else NULL). */
@@ -114,9 +123,11 @@ int sc_proof_result(sc_proof* proof, uint32_t* round_polys, uint32_t* challenges
 /* Copies the prepared input tables (Montgomery, element-major) to host memory. */
 int sc_proof_tables(sc_proof* proof, uint32_t* out);
 
-/* Describes the prepared proof: degree, number of rounds done as fused passes and device
- * bytes held. Any pointer may be NULL. */
-int sc_proof_info(sc_proof* proof, int32_t* degree, int32_t* pass_rounds, uint64_t* device_bytes);
+/* Describes the prepared proof: degree, number of rounds run before the tail kernel, device
+ * bytes held, and the pair count at or below which rounds run point-parallel (the value in
+ * effect, see sc_proof_set_point_parallel). Any pointer may be NULL. */
+int sc_proof_info(sc_proof* proof, int32_t* degree, int32_t* pass_rounds, uint64_t* device_bytes,
+                  int64_t* point_parallel_pairs);
 
 /* Compositions with compiled kernels (a*b, a*b*c, a*b*c + d*e) run them by default; enable = 0
  * forces the interpreted kernels (same results). Returns the compiled composition id that

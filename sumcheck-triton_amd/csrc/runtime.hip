@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdio>
 #include <deque>
 #include <map>
@@ -254,8 +255,9 @@ struct sc_proof {
   ProofState* d_state = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int max_pass_blocks = 2048;
-  bool default_blocks = true;  // grid cap follows the kernel choice (no caller-set value)
   int64_t tail_pairs = 0;
+  int64_t wide_pairs = 0;      // rounds with at most this many pairs run point-parallel
+  bool default_wide = true;    // wide_pairs follows the kernel choice (no caller-set value)
   bool fused_finalize = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -270,11 +272,10 @@ struct sc_proof {
 namespace {
 
 // Pass-kernel grid caps from sweeps on MI355X (a*b*c + d*e at the benchmark sizes; fewer
-// blocks also means fewer partials for the finalize kernel to sum). Compositions of more than
-// 8 tables (and interpreted 255-bit compositions) use the point-parallel kernel, which holds
-// few registers per lane and wants the largest grid (BLS12-381 constraint workload: 22.9 ms at 512 blocks, 19.4 ms at 4096).
-int default_pass_blocks(int L, bool wide) {
-  if (wide) return MAX_PASS_BLOCKS;
+// blocks also means fewer partials for the finalize kernel to sum). The point-parallel kernel
+// holds few registers per lane and always takes the largest grid (BLS12-381 constraint
+// workload: 22.9 ms at 512 blocks, 19.4 ms at 4096).
+int default_pass_blocks(int L) {
   switch (L) {
     case 1: return 2048;
     case 2: return 1024;
@@ -291,10 +292,28 @@ int64_t default_tail_pairs(int L, bool over8) {
   return L == 8 ? 64 : 512;
 }
 
-// Interpreted compositions of more than 8 tables, and every interpreted composition of the
-// 255-bit field (whose per-pair kernels need one wave per SIMD for 2 x 8 working values and 8
-// accumulators of 9 limbs), run as a fold launch plus the point-parallel evaluation.
-bool proof_is_wide(const sc_proof* p) { return p->sid == 0 && (p->tmax > 8 || p->L == 8); }
+// Rounds with at most this many pairs run as a fold launch plus the point-parallel evaluation
+// instead of the per-pair pass kernel. Interpreted compositions of more than 8 tables use it
+// in every round. Interpreted 255-bit compositions use it below 2^20 pairs: their per-pair
+// kernel runs one wave per SIMD (2 x 8 working values and 8 accumulators of 9 limbs), which
+// the point-parallel split beats by 8-30% at num_vars 16-20 (MI355X); above that the per-pair
+// kernel's fused fold saves a write and a read of every table and wins for most compositions.
+// Compiled compositions keep the per-pair kernels (a*b*c + d*e, 255-bit, num_vars 24:
+// 4.3 ms against 6.2 ms all point-parallel).
+int64_t default_wide_pairs(const sc_proof* p) {
+  if (p->sid != 0) return 0;
+  if (p->tmax > 8) return INT64_MAX;
+  return p->L == 8 ? (int64_t(1) << 19) : 0;
+}
+
+// A captured proof graph bakes in the schedule; changing it discards the graph.
+void drop_graph(sc_proof* p) {
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->graph_exec = nullptr;
+  p->graph = nullptr;
+  p->graph_mode = -1;
+}
 
 // First round that runs in the tail kernel.
 int tail_first_round(const sc_proof* p) {
@@ -331,8 +350,7 @@ int enqueue_rounds(sc_proof* p, int mode) {
     a.pad0 = 0;
     uint64_t g = (a.pairs + BLOCK - 1) / BLOCK;
     if (g > (uint64_t)p->max_pass_blocks) g = (uint64_t)p->max_pass_blocks;
-    // Interpreted compositions of more than 8 tables: fold launch + point-parallel evaluation.
-    const bool wide = proof_is_wide(p);
+    const bool wide = (int64_t)a.pairs <= p->wide_pairs;
     if (p->profiling) SC_HIP(hipEventRecord(p->prof_ev[2 * j], s));
     if (wide) {
       if (j > 0) {
@@ -346,7 +364,7 @@ int enqueue_rounds(sc_proof* p, int mode) {
       e.fold = 0;
       e.fused_finalize = 0;
       const uint64_t pairs_per_block = BLOCK / p->pmax;
-      g = std::min<uint64_t>((a.pairs + pairs_per_block - 1) / pairs_per_block, (uint64_t)p->max_pass_blocks);
+      g = std::min<uint64_t>((a.pairs + pairs_per_block - 1) / pairs_per_block, (uint64_t)MAX_PASS_BLOCKS);
       SC_HIP(p->ln->wide_eval(s, p->tmax, p->pmax, e, (unsigned)g));
     } else {
       SC_HIP(p->ln->pass(s, p->sid, p->tmax, p->pmax, a, (unsigned)g));
@@ -515,9 +533,10 @@ int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* co
   p->pmax = p->npts <= 4 ? 4 : 8;
   p->n0 = (uint64_t)1 << num_vars;
   p->tail_pairs = default_tail_pairs(p->L, p->tmax > 8);
-  p->max_pass_blocks = default_pass_blocks(p->L, proof_is_wide(p));
+  p->max_pass_blocks = default_pass_blocks(p->L);
   p->matched_sid = match_static(desc);
   p->sid = p->matched_sid;
+  p->wide_pairs = default_wide_pairs(p);
 
   const uint64_t words = (uint64_t)p->T * p->n0 * p->L;
   auto cleanup = [&](int code) {
@@ -597,20 +616,21 @@ int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* co
 int sc_proof_set_schedule(sc_proof* p, int32_t max_pass_blocks, int64_t tail_pairs, int32_t fused_finalize) {
   if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
   if (fused_finalize < 0 || fused_finalize > 1) return fail(SC_ERR_ARG, "fused_finalize must be 0 or 1");
-  p->fused_finalize = fused_finalize != 0;
   if (max_pass_blocks < 0 || max_pass_blocks > MAX_PASS_BLOCKS)
     return fail(SC_ERR_ARG, "max_pass_blocks must be in [0, 4096]");
   if (tail_pairs < 0) return fail(SC_ERR_ARG, "tail_pairs must be >= 0");
-  p->default_blocks = max_pass_blocks == 0;
-  p->max_pass_blocks = max_pass_blocks ? max_pass_blocks : default_pass_blocks(p->L, proof_is_wide(p));
+  p->fused_finalize = fused_finalize != 0;
+  p->max_pass_blocks = max_pass_blocks ? max_pass_blocks : default_pass_blocks(p->L);
   p->tail_pairs = tail_pairs ? tail_pairs : default_tail_pairs(p->L, p->tmax > 8);
-  if (p->graph_exec) {
-    (void)hipGraphExecDestroy(p->graph_exec);
-    (void)hipGraphDestroy(p->graph);
-    p->graph_exec = nullptr;
-    p->graph = nullptr;
-    p->graph_mode = -1;
-  }
+  drop_graph(p);
+  return SC_OK;
+}
+
+int sc_proof_set_point_parallel(sc_proof* p, int64_t max_pairs) {
+  if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
+  p->default_wide = max_pairs < 0;
+  p->wide_pairs = max_pairs < 0 ? default_wide_pairs(p) : max_pairs;
+  drop_graph(p);
   return SC_OK;
 }
 
@@ -643,12 +663,7 @@ int sc_proof_capture(sc_proof* p, int32_t mode, const uint32_t* explicit_r) {
   SC_HIP(hipSetDevice(p->ctx->device));
   rc = upload_explicit(p, mode, explicit_r);
   if (rc != SC_OK) return rc;
-  if (p->graph_exec) {
-    (void)hipGraphExecDestroy(p->graph_exec);
-    (void)hipGraphDestroy(p->graph);
-    p->graph_exec = nullptr;
-    p->graph = nullptr;
-  }
+  drop_graph(p);
   hipStream_t s = p->ctx->stream;
   const bool prof = p->profiling;
   p->profiling = false;  // no event records inside the graph
@@ -731,26 +746,22 @@ int sc_proof_tables(sc_proof* p, uint32_t* out) {
   return SC_OK;
 }
 
-int sc_proof_info(sc_proof* p, int32_t* degree, int32_t* pass_rounds, uint64_t* device_bytes) {
+int sc_proof_info(sc_proof* p, int32_t* degree, int32_t* pass_rounds, uint64_t* device_bytes,
+                  int64_t* point_parallel_pairs) {
   if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
   if (degree) *degree = p->npts - 1;
   if (pass_rounds) *pass_rounds = tail_first_round(p);
   if (device_bytes) *device_bytes = p->device_bytes;
+  if (point_parallel_pairs) *point_parallel_pairs = p->wide_pairs;
   return SC_OK;
 }
 
 int sc_proof_use_compiled(sc_proof* p, int32_t enable) {
   if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
   const int sid = enable ? p->matched_sid : 0;
-  if (sid != p->sid && p->graph_exec) {
-    (void)hipGraphExecDestroy(p->graph_exec);
-    (void)hipGraphDestroy(p->graph);
-    p->graph_exec = nullptr;
-    p->graph = nullptr;
-    p->graph_mode = -1;
-  }
+  if (sid != p->sid) drop_graph(p);
   p->sid = sid;
-  if (p->default_blocks) p->max_pass_blocks = default_pass_blocks(p->L, proof_is_wide(p));
+  if (p->default_wide) p->wide_pairs = default_wide_pairs(p);
   return p->matched_sid;
 }
 

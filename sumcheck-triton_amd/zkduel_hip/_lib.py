@@ -71,12 +71,13 @@ SIGNATURES = [
      [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ScComposition), ctypes.c_int32,
       ctypes.POINTER(ScInput), ctypes.POINTER(ctypes.c_void_p)]),
     ("sc_proof_set_schedule", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32]),
+    ("sc_proof_set_point_parallel", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
     ("sc_proof_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, u32p]),
     ("sc_proof_capture", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, u32p]),
     ("sc_proof_wait", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
     ("sc_proof_result", ctypes.c_int, [ctypes.c_void_p, u32p, u32p, u64p]),
     ("sc_proof_tables", ctypes.c_int, [ctypes.c_void_p, u32p]),
-    ("sc_proof_info", ctypes.c_int, [ctypes.c_void_p, i32p, i32p, u64p]),
+    ("sc_proof_info", ctypes.c_int, [ctypes.c_void_p, i32p, i32p, u64p, ctypes.POINTER(ctypes.c_int64)]),
     ("sc_proof_use_compiled", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("sc_proof_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("sc_proof_kernel_times", ctypes.c_int,

@@ -6,14 +6,18 @@ Knobs, none of which changes any result bit:
 * ``max_pass_blocks`` — grid cap of the per-round pass kernel (the kernel loops over pairs);
 * ``tail_pairs`` — rounds with at most this many pairs are fused into the single-workgroup tail
   kernel (the "fusion depth": how many final rounds run in one launch);
+* ``wide_pairs`` — rounds with at most this many pairs run as a fold launch plus the
+  point-parallel evaluation instead of the per-pair pass kernel (-1: the library's default);
 * ``items`` (batches) — pairs per thread in a batch tile.
 
-Results are cached per (bit_width, num_tables, degree, num_vars) for the process, and, when
-``path`` is given, in a JSON file, so a serving process can tune once per shape class.
+Results are cached per (bit_width, composition, num_vars, compiled or interpreted kernels) for
+the process, and, when ``path`` is given, in a JSON file, so a serving process can tune once
+per shape class.
 """
 
 from __future__ import annotations
 
+import hashlib
 import json
 import os
 from pathlib import Path
@@ -23,12 +27,15 @@ from .prover import Batch, Proof
 
 PASS_BLOCKS = (512, 1024, 2048, 4096)
 TAIL_PAIRS = (64, 256, 512, 2048)
+WIDE_PAIRS = (-1, 0, 1 << 15, 1 << 17, 1 << 19, 1 << 21, 1 << 40)
 
 _cache: dict[str, dict] = {}
 
 
 def _key(proof: Proof) -> str:
-    return f"{proof.field.bit_width}/{proof.comp.num_tables}/{proof.comp.degree}/{proof.num_vars}"
+    terms = hashlib.sha1(repr(proof.comp.terms()).encode()).hexdigest()[:12]
+    return (f"{proof.field.bit_width}/{proof.comp.num_tables}/{proof.comp.degree}/{proof.num_vars}"
+            f"/{terms}/{'compiled' if proof.compiled else 'interpreted'}")
 
 
 def _load(path: Path | None) -> None:
@@ -58,35 +65,42 @@ def _time(proof: Proof, mode: str, repeats: int) -> float:
     return best
 
 
+def _apply(proof: Proof, choice: dict) -> None:
+    proof.set_schedule(choice["max_pass_blocks"], choice["tail_pairs"], False)
+    proof.set_point_parallel(choice.get("wide_pairs", -1))
+
+
 def tune_proof(proof: Proof, challenge_mode: str = "sha3", repeats: int = 3,
                pass_blocks: Iterable[int] = PASS_BLOCKS, tail_pairs: Iterable[int] = TAIL_PAIRS,
+               wide_pairs: Iterable[int] = WIDE_PAIRS,
                path: str | os.PathLike | None = None) -> dict:
-    """Sets the fastest (max_pass_blocks, tail_pairs) on ``proof`` and returns
-    {"max_pass_blocks", "tail_pairs", "ms", "candidates": {"blocks/tail": ms}}. A cached
-    choice for the same shape is applied without timing."""
+    """Sets the fastest (max_pass_blocks, tail_pairs, wide_pairs) on ``proof`` and returns
+    {"max_pass_blocks", "tail_pairs", "wide_pairs", "ms", "candidates": {"blocks/tail/wide":
+    ms}}. A cached choice for the same shape is applied without timing."""
     p = Path(path) if path is not None else None
     _load(p)
     key = _key(proof)
     if key in _cache:
-        choice = _cache[key]
-        proof.set_schedule(choice["max_pass_blocks"], choice["tail_pairs"], False)
-        return choice
+        _apply(proof, _cache[key])
+        return _cache[key]
+    pass_blocks, tail_pairs, wide_pairs = tuple(pass_blocks), tuple(tail_pairs), tuple(wide_pairs)
     timings: dict[str, float] = {}
-    best = None
-    # the two knobs are nearly independent: sweep the grid cap at the default tail, then the
-    # tail at the best grid cap
-    for blocks in pass_blocks:
-        proof.set_schedule(blocks, 0, False)
-        timings[f"{blocks}/default"] = _time(proof, challenge_mode, repeats)
-    b_best = min(pass_blocks, key=lambda b: timings[f"{b}/default"])
-    for tail in tail_pairs:
-        proof.set_schedule(b_best, tail, False)
-        ms = _time(proof, challenge_mode, repeats)
-        timings[f"{b_best}/{tail}"] = ms
-        if best is None or ms < best[2]:
-            best = (b_best, tail, ms)
-    choice = {"max_pass_blocks": best[0], "tail_pairs": best[1], "ms": best[2], "candidates": timings}
-    proof.set_schedule(best[0], best[1], False)
+
+    def timed(blocks: int, tail: int, wide: int) -> float:
+        name = f"{blocks}/{tail}/{wide}"
+        if name not in timings:
+            _apply(proof, {"max_pass_blocks": blocks, "tail_pairs": tail, "wide_pairs": wide})
+            timings[name] = _time(proof, challenge_mode, repeats)
+        return timings[name]
+
+    # the knobs are nearly independent: sweep the kernel choice at the default grid and tail,
+    # then the grid cap, then the tail, each at the best of the ones before
+    w_best = min(wide_pairs, key=lambda w: timed(0, 0, w))
+    b_best = min(pass_blocks, key=lambda b: timed(b, 0, w_best))
+    t_best = min(tail_pairs, key=lambda t: timed(b_best, t, w_best))
+    choice = {"max_pass_blocks": b_best, "tail_pairs": t_best, "wide_pairs": w_best,
+              "ms": timings[f"{b_best}/{t_best}/{w_best}"], "candidates": timings}
+    _apply(proof, choice)
     _cache[key] = choice
     _store(p)
     return choice

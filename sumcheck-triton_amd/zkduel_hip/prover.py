@@ -200,6 +200,7 @@ class Proof:
                                             self.num_vars, ctypes.byref(inp), ctypes.byref(handle)))
         self.handle = handle
         self._keep = None  # host input has been copied to the device
+        self.compiled = True  # matching compiled kernels in use (use_compiled)
 
     def close(self) -> None:
         if self.handle:
@@ -232,6 +233,11 @@ class Proof:
         _lib.check(self.lib.sc_proof_set_schedule(self.handle, int(max_pass_blocks), int(tail_pairs),
                                                   1 if fused_finalize else 0))
 
+    def set_point_parallel(self, max_pairs: int = -1) -> None:
+        """Rounds with at most ``max_pairs`` pairs run as fold + point-parallel evaluation
+        instead of the per-pair pass kernel (0: never; negative: the default rule)."""
+        _lib.check(self.lib.sc_proof_set_point_parallel(self.handle, int(max_pairs)))
+
     def run(self, challenge_mode: str = "sha3", challenges=None) -> None:
         mode = _lib.CHALLENGE_MODES[challenge_mode]
         r = self._explicit(mode, challenges)
@@ -263,9 +269,11 @@ class Proof:
         return out
 
     def info(self) -> dict:
-        d, pr, nb = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_uint64()
-        _lib.check(self.lib.sc_proof_info(self.handle, ctypes.byref(d), ctypes.byref(pr), ctypes.byref(nb)))
-        return {"degree": d.value, "pass_rounds": pr.value, "device_bytes": nb.value}
+        d, pr, nb, wp = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_uint64(), ctypes.c_int64()
+        _lib.check(self.lib.sc_proof_info(self.handle, ctypes.byref(d), ctypes.byref(pr), ctypes.byref(nb),
+                                          ctypes.byref(wp)))
+        return {"degree": d.value, "pass_rounds": pr.value, "device_bytes": nb.value,
+                "point_parallel_pairs": wp.value}
 
     def use_compiled(self, enable: bool = True) -> int:
         """Use the compiled kernels of a matching composition (default) or force the
@@ -273,6 +281,7 @@ class Proof:
         rc = self.lib.sc_proof_use_compiled(self.handle, 1 if enable else 0)
         if rc < 0:
             _lib.check(-rc)
+        self.compiled = bool(enable)
         return rc
 
     def set_profiling(self, enable: bool = True) -> None:

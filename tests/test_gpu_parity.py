@@ -207,6 +207,50 @@ def test_wide_compositions_match_c_oracle(ctx, bw):
 
 
 @pytest.mark.parametrize("bw", WIDTHS)
+def test_point_parallel_rounds_match_c_oracle(ctx, bw):
+    """The per-round kernel choice (per-pair pass kernel with the fold fused in, or a fold
+    launch plus the point-parallel evaluation): no rounds, the rounds below 2^6 and 2^9
+    pairs, every round, and the library default, for compiled a*b*c + d*e and for
+    interpreted sums of products over up to 8 tables with field coefficients and a
+    selector, against the C oracle; with fused finalize and under graph replay too."""
+    from oracle import cbind
+
+    f = get_field(bw)
+    rng = random.Random(700 + bw)
+    comps = [(Composition.from_spec(get_spec("poly_abc_plus_de")), 0)]
+    for T, degs, nterms in ((8, (2, 6), 4), (3, (4, 8), 2), (6, (2, 4), 5)):
+        terms = [(rng.randrange(f.modulus), [rng.randrange(T) for _ in range(rng.randrange(*degs))])
+                 for _ in range(nterms)]
+        comps.append((Composition.from_terms(T, terms), 1 << (T - 1)))
+    nv = 12
+    for comp, mask in comps:
+        want_g, want_r, _ = cbind.prove(bw, comp.num_tables, _oracle_terms(comp), nv, seed=43,
+                                        selector_mask=mask)
+        with Proof(ctx, bw, comp, nv, input="uniform", seed=43, selector_mask=mask) as pr:
+            for compiled in (True, False):
+                pr.use_compiled(compiled)
+                for wide, fused in ((0, False), (1 << 6, False), (1 << 9, True), (1 << 40, False),
+                                    (-1, True)):
+                    say("point-parallel", bw, comp.num_tables, compiled, wide, fused)
+                    pr.set_schedule(0, 16, fused)
+                    pr.set_point_parallel(wide)
+                    pr.run("sha3")
+                    pr.wait()
+                    g, r, _cs = pr.result()
+                    assert np.array_equal(g, want_g) and np.array_equal(r, want_r), (compiled, wide, fused)
+            pr.set_point_parallel(1 << 9)
+            pr.capture("sha3")
+            for k in range(2):
+                say("point-parallel graph replay", k)
+                pr.run("sha3")
+                pr.wait()
+                g, r, _cs = pr.result()
+                assert np.array_equal(g, want_g) and np.array_equal(r, want_r)
+                pr.run("fixed")
+                pr.wait()
+
+
+@pytest.mark.parametrize("bw", WIDTHS)
 def test_final_point_and_verifier(ctx, bw):
     """Final-point step: r_final follows the transcript and every table's final value is its
     multilinear extension at (r_0, ..., r_final), computed here from the input tables; the
