@@ -176,8 +176,15 @@ int sc_transcript_challenge(sc_context* ctx, int32_t bit_width, int32_t round, c
 typedef struct sc_batch sc_batch;
 int sc_batch_create(sc_context* ctx, int32_t bit_width, int32_t n, const sc_composition* comps,
                     const int32_t* num_vars, const sc_input* inputs, sc_batch** out);
-/* Pairs per thread in a tile (tile = 256 * items pairs); default 4. */
+/* Pairs per thread in a tile (tile = 256 * items pairs; 256 / (degree + 1 rounded up to 4 or
+ * 8) * items pairs in point-parallel rounds); default 4. */
 int sc_batch_set_tile(sc_batch* batch, int32_t items_per_thread);
+/* Instance rounds with at most max_pairs pairs run point-parallel (a fold launch plus an
+ * evaluation launch with one lane per pair and point, as sc_proof_set_point_parallel for one
+ * proof); 0 never, negative the default (every round when an instance of the batch has more
+ * than 8 tables, none otherwise). Results do not depend on it.
+ * No counterpart in the reference (one instance per run). */
+int sc_batch_set_point_parallel(sc_batch* batch, int64_t max_pairs);
 /* challenge_mode SC_CHALLENGE_FIXED or SC_CHALLENGE_SHA3; enqueues without waiting. */
 int sc_batch_run(sc_batch* batch, int32_t challenge_mode);
 int sc_batch_capture(sc_batch* batch, int32_t challenge_mode);

@@ -587,59 +587,60 @@ __global__ void __launch_bounds__(TB) tail_kernel(TailArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Wide compositions (more than 8 tables, interpreted): the per-pair kernels above would hold
-// 2 x tables working values per thread and spill. These rounds run as two launches instead:
-// wide_fold_kernel writes round j's tables (fold of round j-1 with r_{j-1}); wide_eval_kernel
-// gives each pair G = PMAX lanes, lane t evaluating the composition at point t only, so a
-// lane holds one value per table and one accumulator. Same partials layout as pass_kernel.
+// Point-parallel rounds. The per-pair kernels above hold 2 x tables working values and PMAX
+// accumulators per thread: with more than 8 tables they would spill, and at 255 bits they run
+// one wave per SIMD. Such rounds run as two launches instead: a fold launch writes round j's
+// tables (fold of round j-1 with r_{j-1}); the evaluation gives each pair G = PMAX lanes, lane
+// t evaluating the composition at point t only, so a lane holds one value per table and one
+// accumulator. Same partials layout as pass_kernel (batch_pass_kernel for the batched forms).
 // ---------------------------------------------------------------------------------------------
+// Elements idx = begin, begin + stride, ... < end of round j's tables (T tables of n_new
+// entries, idx = table * n_new + entry), each folded from two entries of round j-1's src.
 template <class F>
-__global__ void __launch_bounds__(BLOCK) wide_fold_kernel(PassArgs a) {
+__device__ __forceinline__ void wide_fold_range(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                                uint64_t src_n, bool src_mont, const typename Work<F>::V& r,
+                                                uint64_t begin, uint64_t end, uint64_t stride) {
   using W = Work<F>;
   constexpr int L = F::L;
-  const int T = a.comp->num_tables;
-  const uint64_t n_new = a.src_n >> 1;  // a power of two
+  const uint64_t n_new = src_n >> 1;  // a power of two
   const int shift = __builtin_ctzll(n_new);
-  const uint64_t total = n_new * (uint64_t)T;
-  const bool src_mont = a.round <= 1;
-  const typename W::V r = W::from_mont(a.st->r[a.round - 1]);
-  for (uint64_t idx = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; idx < total;
-       idx += (uint64_t)gridDim.x * BLOCK) {
+  for (uint64_t idx = begin; idx < end; idx += stride) {
     const uint64_t v = idx >> shift, i = idx & (n_new - 1);
     uint32_t w[2 * L];
-    load_words<2 * L>(w, a.src + (v * a.src_n + 2 * i) * L);
+    load_words<2 * L>(w, src + (v * src_n + 2 * i) * L);
     const typename W::V lo = src_mont ? W::from_mont(w) : W::from_stored(w);
     const typename W::V hi = src_mont ? W::from_mont(w + L) : W::from_stored(w + L);
     uint32_t y[L];
     W::store(y, W::fold(lo, hi, r));
-    uint32_t* d = a.dst + (v * n_new + i) * L;
+    uint32_t* d = dst + (v * n_new + i) * L;
 #pragma unroll
     for (int k = 0; k < L; ++k) d[k] = y[k];
   }
 }
 
+// Point-parallel evaluation of pairs begin + threadIdx.x / G, then every stride pairs, below
+// end: lane t = threadIdx.x % G evaluates the composition at point t only. Writes the block's
+// sums for t < npts to part[t * L + k].
 template <class F, int TMAX, int G>
-__global__ void __launch_bounds__(BLOCK) wide_eval_kernel(PassArgs a) {
+__device__ __forceinline__ void wide_eval_block(const uint32_t* __restrict__ src, uint64_t src_n, bool src_mont,
+                                                uint64_t begin, uint64_t end, uint64_t stride,
+                                                const CompDesc* __restrict__ cd, uint32_t* __restrict__ part,
+                                                uint32_t* s_red) {
   using W = Work<F>;
   constexpr int L = F::L;
-  constexpr int PAIRS = BLOCK / G;  // pairs per block per sweep
-  __shared__ uint32_t s_red[(BLOCK / 64) * G * L];
-  const CompDesc* __restrict__ cd = a.comp;
   const int T = cd->num_tables;
   const int K = cd->num_terms;
   const int npts = cd->npts;
   const int t = threadIdx.x % G;
-  const bool src_mont = a.round == 0;
   typename W::Acc acc;
   acc.clear();
-  const uint64_t stride = (uint64_t)gridDim.x * PAIRS;
-  for (uint64_t i = (uint64_t)blockIdx.x * PAIRS + threadIdx.x / G; i < a.pairs; i += stride) {
+  for (uint64_t i = begin + threadIdx.x / G; i < end; i += stride) {
     typename W::V x[TMAX];
     static_for<TMAX>([&](auto vi) {
       constexpr int v = decltype(vi)::value;
       if (v < T) {
         uint32_t w[2 * L];
-        load_words<2 * L>(w, a.src + ((uint64_t)v * a.src_n + 2 * i) * L);
+        load_words<2 * L>(w, src + ((uint64_t)v * src_n + 2 * i) * L);
         const typename W::V lo = src_mont ? W::from_mont(w) : W::from_stored(w);
         const typename W::V hi = src_mont ? W::from_mont(w + L) : W::from_stored(w + L);
         x[v] = W::at(lo, W::diff(hi, lo), t);
@@ -671,8 +672,24 @@ __global__ void __launch_bounds__(BLOCK) wide_eval_kernel(PassArgs a) {
 #pragma unroll
     for (int w = 1; w < BLOCK / 64; ++w) s2 = add<F>(s2, load_elem<L>(s_red + (w * G + threadIdx.x) * L));
 #pragma unroll
-    for (int k = 0; k < L; ++k) a.partials[((uint64_t)blockIdx.x * MAX_POINTS + threadIdx.x) * L + k] = s2.v[k];
+    for (int k = 0; k < L; ++k) part[threadIdx.x * L + k] = s2.v[k];
   }
+}
+
+template <class F>
+__global__ void __launch_bounds__(BLOCK) wide_fold_kernel(PassArgs a) {
+  const uint64_t total = (a.src_n >> 1) * (uint64_t)a.comp->num_tables;
+  wide_fold_range<F>(a.src, a.dst, a.src_n, a.round <= 1, Work<F>::from_mont(a.st->r[a.round - 1]),
+                     (uint64_t)blockIdx.x * BLOCK + threadIdx.x, total, (uint64_t)gridDim.x * BLOCK);
+}
+
+template <class F, int TMAX, int G>
+__global__ void __launch_bounds__(BLOCK) wide_eval_kernel(PassArgs a) {
+  constexpr int PAIRS = BLOCK / G;  // pairs per block per sweep
+  __shared__ uint32_t s_red[(BLOCK / 64) * G * F::L];
+  wide_eval_block<F, TMAX, G>(a.src, a.src_n, a.round == 0, (uint64_t)blockIdx.x * PAIRS, a.pairs,
+                              (uint64_t)gridDim.x * PAIRS, a.comp,
+                              a.partials + (uint64_t)blockIdx.x * MAX_POINTS * F::L, s_red);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -732,6 +749,42 @@ __global__ void __launch_bounds__(BLOCK) batch_pass_kernel(BatchArgs a) {
   if (threadIdx.x < npts * L) {
     a.partials[(uint64_t)blockIdx.x * (MAX_POINTS * MAX_LIMBS) + threadIdx.x] = s_g[threadIdx.x];
   }
+}
+
+// Point-parallel form of a stage (entries whose round runs point-parallel). Fold: block b
+// folds elements of tile (b - tile_begin) of its entry's round (BLOCK * items elements of
+// the T * n_round table entries per tile); entries have round >= 1.
+template <class F>
+__global__ void __launch_bounds__(BLOCK) batch_wide_fold_kernel(BatchArgs a) {
+  const StageEntry ent = a.entries[find_entry(a.entries, a.n_entries, blockIdx.x)];
+  const InstDesc& in = a.insts[ent.inst];
+  const int j = (int)ent.round;
+  const uint32_t* src = j <= 1 ? in.input : (((j - 1) & 1) ? in.buf_a : in.buf_b);
+  uint32_t* dst = (j & 1) ? in.buf_a : in.buf_b;
+  const uint64_t src_n = in.n0 >> (j - 1);
+  const uint64_t total = (src_n >> 1) * (uint64_t)in.comp->num_tables;
+  const uint64_t span = (uint64_t)BLOCK * a.items;
+  const uint64_t begin = (uint64_t)(blockIdx.x - ent.tile_begin) * span;
+  wide_fold_range<F>(src, dst, src_n, j <= 1, Work<F>::from_mont(in.st->r[j - 1]), begin + threadIdx.x,
+                     begin + span < total ? begin + span : total, BLOCK);
+}
+
+// Evaluation: block b (tile index b + tile_base in the stage's partials) evaluates
+// (BLOCK / G) * items pairs of its entry's round and writes the tile's partial.
+template <class F, int TMAX, int G>
+__global__ void __launch_bounds__(BLOCK) batch_wide_eval_kernel(BatchArgs a) {
+  constexpr int PAIRS = BLOCK / G;
+  __shared__ uint32_t s_red[(BLOCK / 64) * G * F::L];
+  const uint32_t b = blockIdx.x + a.tile_base;
+  const StageEntry ent = a.entries[find_entry(a.entries, a.n_entries, b)];
+  const InstDesc& in = a.insts[ent.inst];
+  const int j = (int)ent.round;
+  const uint32_t* src = j == 0 ? in.input : ((j & 1) ? in.buf_a : in.buf_b);
+  const uint64_t pairs = in.n0 >> (j + 1);
+  const uint64_t span = (uint64_t)PAIRS * a.items;
+  const uint64_t begin = (uint64_t)(b - ent.tile_begin) * span;
+  wide_eval_block<F, TMAX, G>(src, in.n0 >> j, j == 0, begin, begin + span < pairs ? begin + span : pairs, PAIRS,
+                              in.comp, a.partials + (uint64_t)b * (MAX_POINTS * MAX_LIMBS), s_red);
 }
 
 // One block per entry: the entry's instance runs all its remaining rounds from entry.round in

@@ -24,12 +24,16 @@ struct Launchers {
   hipError_t (*batch_finalize)(hipStream_t, int pmax, const BatchArgs&, unsigned entries);
   // one block per entry running all of its instance's remaining rounds
   hipError_t (*batch_tail)(hipStream_t, int tmax, int pmax, const BatchArgs&, unsigned entries);
-  // wide interpreted compositions (tmax > 8): fold launch (round >= 1), then a point-parallel
-  // evaluation launch with pmax lanes per pair
+  // point-parallel rounds: fold launch (round >= 1), then an evaluation launch with pmax lanes
+  // per pair
   hipError_t (*wide_fold)(hipStream_t, const PassArgs&, unsigned grid);
   hipError_t (*wide_eval)(hipStream_t, int tmax, int pmax, const PassArgs&, unsigned grid);
   // limb-major input tables -> element-major (n entries per table, a power of two)
   hipError_t (*limb_to_element)(hipStream_t, const uint32_t* in, uint32_t* out, uint64_t n, int num_tables);
+  // point-parallel form of a batched stage: fold tiles, then evaluation tiles (args.tile_base
+  // = the stage's per-pair tiles, so the evaluation partials follow theirs)
+  hipError_t (*batch_wide_fold)(hipStream_t, const BatchArgs&, unsigned tiles);
+  hipError_t (*batch_wide_eval)(hipStream_t, int tmax, int pmax, const BatchArgs&, unsigned tiles);
 };
 
 const Launchers& launchers_p32();

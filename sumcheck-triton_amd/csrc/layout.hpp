@@ -173,6 +173,8 @@ struct BatchArgs {
   uint32_t items;             // pairs per thread in a tile: a tile is BLOCK * items pairs
   uint32_t* partials;         // [tiles][MAX_POINTS][MAX_LIMBS]
   const FieldConsts* fc;
+  uint32_t tile_base;         // point-parallel evaluation: block b is tile b + tile_base
+  uint32_t pad0;
 };
 
 }  // namespace sc

@@ -306,6 +306,87 @@ int64_t default_wide_pairs(const sc_proof* p) {
   return p->L == 8 ? (int64_t(1) << 19) : 0;
 }
 
+// Batches and servers: point-parallel rounds only where the per-pair kernels would spill
+// (more than 8 tables). A stage's many instances already fill the GPU, which is what the
+// point-parallel split buys a single small proof; at 255 bits, 256 instances of num_vars
+// 10-22 (MI355X): 101 ms per-pair, 112 ms with rounds below 2^20 pairs point-parallel,
+// 118 ms all point-parallel.
+int64_t default_batch_wide_pairs(int tmax) { return tmax > 8 ? INT64_MAX : 0; }
+
+// Work lists of one batched stage (one round of each listed instance). Per-pair entries own
+// tiles [0, pass_tiles) of the stage's partials and point-parallel entries the next
+// eval_tiles; fold entries (point-parallel rounds >= 1) number their tiles apart. On the
+// device the lists sit as [pass][eval][fold], so one finalize launch covers [pass][eval].
+struct StagePlan {
+  std::vector<StageEntry> pass, eval, fold;
+  uint64_t pass_tiles = 0, eval_tiles = 0, fold_tiles = 0;
+
+  // Partial tiles the round adds: pass tiles of BLOCK * items pairs, or evaluation tiles of
+  // (BLOCK / pmax) * items pairs.
+  static uint64_t tiles(uint64_t pairs, bool wide, uint32_t items, int pmax) {
+    const uint64_t span = (wide ? (uint64_t)(BLOCK / pmax) : (uint64_t)BLOCK) * items;
+    return (pairs + span - 1) / span;
+  }
+  void add(uint32_t inst, uint32_t round, uint64_t pairs, int num_tables, bool wide, uint32_t items, int pmax) {
+    const uint32_t t = (uint32_t)tiles(pairs, wide, items, pmax);
+    if (!wide) {
+      pass.push_back(StageEntry{inst, round, (uint32_t)pass_tiles, t});
+      pass_tiles += t;
+      return;
+    }
+    eval.push_back(StageEntry{inst, round, (uint32_t)eval_tiles, t});  // rebased in layout()
+    eval_tiles += t;
+    if (round > 0) {
+      const uint64_t elems = 2 * pairs * (uint64_t)num_tables, span = (uint64_t)BLOCK * items;
+      const uint32_t f = (uint32_t)((elems + span - 1) / span);
+      fold.push_back(StageEntry{inst, round, (uint32_t)fold_tiles, f});
+      fold_tiles += f;
+    }
+  }
+  uint64_t partial_tiles() const { return pass_tiles + eval_tiles; }
+  // Appends [pass][eval][fold] to out, evaluation tiles after the pass tiles.
+  void layout(std::vector<StageEntry>& out) const {
+    out.insert(out.end(), pass.begin(), pass.end());
+    for (StageEntry e : eval) {
+      e.tile_begin += (uint32_t)pass_tiles;
+      out.push_back(e);
+    }
+    out.insert(out.end(), fold.begin(), fold.end());
+  }
+  size_t size() const { return pass.size() + eval.size() + fold.size(); }
+};
+
+// Launches one stage laid out by StagePlan::layout at d (device), then its finalize.
+int enqueue_stage(const Launchers& ln, hipStream_t s, BatchArgs a, int tmax, int pmax, const StageEntry* d,
+                  uint32_t n_pass, uint32_t n_eval, uint32_t n_fold, uint64_t pass_tiles, uint64_t eval_tiles,
+                  uint64_t fold_tiles) {
+  a.tile_base = 0;
+  a.pad0 = 0;
+  if (n_pass) {
+    a.entries = d;
+    a.n_entries = n_pass;
+    SC_HIP(ln.batch_pass(s, tmax, pmax, a, (unsigned)pass_tiles));
+  }
+  if (n_fold) {
+    a.entries = d + n_pass + n_eval;
+    a.n_entries = n_fold;
+    SC_HIP(ln.batch_wide_fold(s, a, (unsigned)fold_tiles));
+  }
+  if (n_eval) {
+    a.entries = d + n_pass;
+    a.n_entries = n_eval;
+    a.tile_base = (uint32_t)pass_tiles;
+    SC_HIP(ln.batch_wide_eval(s, tmax, pmax, a, (unsigned)eval_tiles));
+    a.tile_base = 0;
+  }
+  if (n_pass + n_eval) {
+    a.entries = d;
+    a.n_entries = n_pass + n_eval;
+    SC_HIP(ln.batch_finalize(s, pmax, a, n_pass + n_eval));
+  }
+  return SC_OK;
+}
+
 // A captured proof graph bakes in the schedule; changing it discards the graph.
 void drop_graph(sc_proof* p) {
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
@@ -893,7 +974,7 @@ struct sc_batch {
   sc_context* ctx = nullptr;
   int fidx = 0, L = 0, n = 0, tmax = 0, pmax = 0, max_rounds = 0;
   const Launchers* ln = nullptr;
-  std::vector<int> num_vars, npts;
+  std::vector<int> num_vars, npts, num_tables;
   std::vector<uint64_t> n0;
   uint8_t* pool = nullptr;
   uint64_t pool_bytes = 0;
@@ -902,7 +983,10 @@ struct sc_batch {
   ProofState* d_states = nullptr;
   StageEntry* d_entries = nullptr;
   uint32_t* d_partials = nullptr;
-  std::vector<uint32_t> stage_offset, stage_entries, stage_tiles;
+  // Stage s: its lists start at stage_offset[s] in d_entries, laid out as StagePlan::layout.
+  std::vector<uint32_t> stage_offset, stage_pass, stage_eval, stage_fold;
+  std::vector<uint64_t> stage_pass_tiles, stage_eval_tiles, stage_fold_tiles;
+  int64_t wide_pairs = 0;  // rounds with at most this many pairs run point-parallel
   // Each instance's last rounds (from tail_round[i] on, at most tail_pairs pairs) run as one
   // tail entry of the last stage: one workgroup loops over them, as a single proof's tail
   // kernel does. All tails share that one launch (a tail launch per stage measured slower:
@@ -949,26 +1033,34 @@ int build_stages(sc_batch* b) {
   const int last = b->num_stages - 1;
   const int ns = b->num_stages;
   b->stage_offset.assign(ns, 0);
-  b->stage_entries.assign(ns, 0);
-  b->stage_tiles.assign(ns, 0);
+  b->stage_pass.assign(ns, 0);
+  b->stage_eval.assign(ns, 0);
+  b->stage_fold.assign(ns, 0);
+  b->stage_pass_tiles.assign(ns, 0);
+  b->stage_eval_tiles.assign(ns, 0);
+  b->stage_fold_tiles.assign(ns, 0);
   b->tail_offset.assign(ns, 0);
   b->tail_entries.assign(ns, 0);
   uint64_t max_tiles = 1;
-  const uint64_t tile_pairs = (uint64_t)BLOCK * b->items;
   for (int s = 0; s < ns; ++s) {
     b->stage_offset[s] = (uint32_t)all.size();
-    uint64_t tiles = 0;
+    StagePlan plan;
     for (int i = 0; i < b->n; ++i) {
       if (s >= b->tail_round[i]) continue;
       const uint64_t pairs = b->n0[i] >> (s + 1);
-      const uint64_t t = (pairs + tile_pairs - 1) / tile_pairs;
-      if (tiles + t > 0x7FFFFFFFull) return fail(SC_ERR_UNSUPPORTED, "batch stage has too many tiles");
-      all.push_back(StageEntry{(uint32_t)i, (uint32_t)s, (uint32_t)tiles, (uint32_t)t});
-      tiles += t;
+      plan.add((uint32_t)i, (uint32_t)s, pairs, b->num_tables[i], (int64_t)pairs <= b->wide_pairs, b->items,
+               b->pmax);
+      if (plan.partial_tiles() > 0x7FFFFFFFull || plan.fold_tiles > 0x7FFFFFFFull)
+        return fail(SC_ERR_UNSUPPORTED, "batch stage has too many tiles");
     }
-    b->stage_entries[s] = (uint32_t)all.size() - b->stage_offset[s];
-    b->stage_tiles[s] = (uint32_t)tiles;
-    if (tiles > max_tiles) max_tiles = tiles;
+    plan.layout(all);
+    b->stage_pass[s] = (uint32_t)plan.pass.size();
+    b->stage_eval[s] = (uint32_t)plan.eval.size();
+    b->stage_fold[s] = (uint32_t)plan.fold.size();
+    b->stage_pass_tiles[s] = plan.pass_tiles;
+    b->stage_eval_tiles[s] = plan.eval_tiles;
+    b->stage_fold_tiles[s] = plan.fold_tiles;
+    max_tiles = std::max(max_tiles, plan.partial_tiles());
     b->tail_offset[s] = (uint32_t)all.size();
     if (s == last) {
       for (int i = 0; i < b->n; ++i) all.push_back(StageEntry{(uint32_t)i, (uint32_t)b->tail_round[i], 0u, 0u});
@@ -1006,12 +1098,12 @@ int enqueue_batch(sc_batch* b) {
     a.items = b->items;
     a.partials = b->d_partials;
     a.fc = b->ctx->d_fc[b->fidx];
-    if (b->stage_entries[st] != 0) {
-      a.entries = b->d_entries + b->stage_offset[st];
-      a.n_entries = b->stage_entries[st];
-      SC_HIP(b->ln->batch_pass(s, b->tmax, b->pmax, a, b->stage_tiles[st]));
-      SC_HIP(b->ln->batch_finalize(s, b->pmax, a, b->stage_entries[st]));
-    }
+    a.tile_base = 0;
+    a.pad0 = 0;
+    int rc = enqueue_stage(*b->ln, s, a, b->tmax, b->pmax, b->d_entries + b->stage_offset[st], b->stage_pass[st],
+                           b->stage_eval[st], b->stage_fold[st], b->stage_pass_tiles[st], b->stage_eval_tiles[st],
+                           b->stage_fold_tiles[st]);
+    if (rc != SC_OK) return rc;
     if (b->tail_entries[st] != 0) {
       a.entries = b->d_entries + b->tail_offset[st];
       a.n_entries = b->tail_entries[st];
@@ -1043,6 +1135,7 @@ int sc_batch_create(sc_context* ctx, int32_t bit_width, int32_t n, const sc_comp
   b->n = n;
   b->num_vars.resize(n);
   b->npts.resize(n);
+  b->num_tables.resize(n);
   b->n0.resize(n);
   b->tail_round.resize(n);
   int maxT = 0, maxP = 0;
@@ -1066,6 +1159,7 @@ int sc_batch_create(sc_context* ctx, int32_t bit_width, int32_t n, const sc_comp
     }
     b->num_vars[i] = num_vars[i];
     b->npts[i] = descs[i].npts;
+    b->num_tables[i] = descs[i].num_tables;
     b->n0[i] = (uint64_t)1 << num_vars[i];
     maxT = std::max(maxT, descs[i].num_tables);
     maxP = std::max(maxP, descs[i].npts);
@@ -1076,6 +1170,7 @@ int sc_batch_create(sc_context* ctx, int32_t bit_width, int32_t n, const sc_comp
   }
   b->tmax = pick_tmax(maxT);
   b->pmax = maxP <= 4 ? 4 : 8;
+  b->wide_pairs = default_batch_wide_pairs(b->tmax);
   for (int i = 0; i < n; ++i) {  // first round whose pairs fit the tail (as for a single proof)
     const int64_t tp = default_tail_pairs(b->L, b->tmax > 8);
     int j = 0;
@@ -1127,6 +1222,15 @@ int sc_batch_create(sc_context* ctx, int32_t bit_width, int32_t n, const sc_comp
   }
   *out = b;
   return SC_OK;
+}
+
+int sc_batch_set_point_parallel(sc_batch* b, int64_t max_pairs) {
+  if (b == nullptr) return fail(SC_ERR_ARG, "batch must not be NULL");
+  const int64_t w = max_pairs < 0 ? default_batch_wide_pairs(b->tmax) : max_pairs;
+  if (w == b->wide_pairs) return SC_OK;
+  b->wide_pairs = w;
+  SC_HIP(hipStreamSynchronize(b->ctx->stream));
+  return build_stages(b);
 }
 
 int sc_batch_set_tile(sc_batch* b, int32_t items_per_thread) {
@@ -1220,7 +1324,7 @@ int sc_batch_info(sc_batch* b, uint64_t* device_bytes, int32_t* stages, uint64_t
   if (stages) *stages = b->num_stages;
   if (tiles) {
     uint64_t t = 0;
-    for (uint32_t x : b->stage_tiles) t += x;
+    for (int s = 0; s < b->num_stages; ++s) t += b->stage_pass_tiles[s] + b->stage_eval_tiles[s];
     *tiles = t;
   }
   return SC_OK;
@@ -1320,7 +1424,7 @@ struct sc_server {
   InstDesc* d_insts = nullptr;       // [max_active]
   CompDesc* d_comps = nullptr;       // [max_active]
   ProofState* d_states = nullptr;    // [max_active]
-  StageEntry* d_entries = nullptr;   // [max_active]
+  StageEntry* d_entries = nullptr;   // [2 * max_active]: a step's lists and tails
   uint32_t* d_partials = nullptr;    // [max_tiles][MAX_POINTS][MAX_LIMBS]
   uint64_t max_tiles = 0;
   std::vector<int> free_slots;
@@ -1444,13 +1548,13 @@ int sc_server_create(sc_context* ctx, uint64_t hbm_budget_bytes, int32_t max_act
   SC_HIP_S(hipMalloc(&sv->d_insts, sizeof(InstDesc) * max_active));
   SC_HIP_S(hipMalloc(&sv->d_comps, sizeof(CompDesc) * max_active));
   SC_HIP_S(hipMalloc(&sv->d_states, sizeof(ProofState) * max_active));
-  SC_HIP_S(hipMalloc(&sv->d_entries, sizeof(StageEntry) * max_active));
+  SC_HIP_S(hipMalloc(&sv->d_entries, sizeof(StageEntry) * 2 * max_active));
   SC_HIP_S(hipMalloc(&sv->d_partials, sv->max_tiles * MAX_POINTS * MAX_LIMBS * 4));
   sv->staging.resize(4);
   for (auto& st : sv->staging) {
     SC_HIP_S(hipHostMalloc(&st.insts, sizeof(InstDesc) * max_active, hipHostMallocDefault));
     SC_HIP_S(hipHostMalloc(&st.comps, sizeof(CompDesc) * max_active, hipHostMallocDefault));
-    SC_HIP_S(hipHostMalloc(&st.entries, sizeof(StageEntry) * max_active, hipHostMallocDefault));
+    SC_HIP_S(hipHostMalloc(&st.entries, sizeof(StageEntry) * 2 * max_active, hipHostMallocDefault));
     SC_HIP_S(hipEventCreateWithFlags(&st.ev, hipEventDisableTiming));
   }
 #undef SC_HIP_S
@@ -1557,32 +1661,41 @@ int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
   int staged = 0;
   bool any = false;
   for (int fidx = 0; fidx < 4; ++fidx) {
-    // pass entries (one round each) first, then tail entries: requests whose current round
-    // has at most the tail's pairs finish all their remaining rounds in this step
-    std::vector<StageEntry> ents, tails;
+    std::vector<int> slots;
     int maxT = 0, maxP = 0;
-    uint64_t tiles = 0;
-    const uint64_t tile_pairs = (uint64_t)BLOCK * sv->items;
     for (int slot = 0; slot < sv->max_active; ++slot) {
       const int64_t id = sv->slot_req[slot];
       if (id == 0) continue;
-      Request& r = sv->reqs[id];
+      const Request& r = sv->reqs[id];
       if (r.state != REQ_RUNNING || r.fidx != fidx) continue;
+      slots.push_back(slot);
+      maxT = std::max(maxT, r.desc.num_tables);
+      maxP = std::max(maxP, r.npts);
+    }
+    if (slots.empty()) continue;
+    const int tmax = pick_tmax(maxT), pmax = maxP <= 4 ? 4 : 8;
+    const int64_t wide_pairs = default_batch_wide_pairs(tmax);
+    // one round of each request (per-pair or point-parallel), then tail entries: requests
+    // whose current round has at most the tail's pairs finish all their remaining rounds
+    StagePlan plan;
+    std::vector<StageEntry> tails;
+    for (int slot : slots) {
+      const Request& r = sv->reqs[sv->slot_req[slot]];
       const uint64_t pairs = ((uint64_t)1 << r.num_vars) >> (r.round + 1);
       const int64_t tail_pairs = default_tail_pairs(r.L, pick_tmax(r.desc.num_tables) > 8);
       if ((int64_t)pairs <= tail_pairs) {
         tails.push_back(StageEntry{(uint32_t)slot, (uint32_t)r.round, 0u, 0u});
-      } else {
-        const uint64_t t = (pairs + tile_pairs - 1) / tile_pairs;
-        if (tiles + t > sv->max_tiles) continue;  // runs in a later step
-        ents.push_back(StageEntry{(uint32_t)slot, (uint32_t)r.round, (uint32_t)tiles, (uint32_t)t});
-        tiles += t;
+        continue;
       }
-      maxT = std::max(maxT, r.desc.num_tables);
-      maxP = std::max(maxP, r.npts);
+      const bool wide = (int64_t)pairs <= wide_pairs;
+      if (plan.partial_tiles() + StagePlan::tiles(pairs, wide, sv->items, pmax) > sv->max_tiles)
+        continue;  // runs in a later step
+      plan.add((uint32_t)slot, (uint32_t)r.round, pairs, r.desc.num_tables, wide, sv->items, pmax);
     }
-    if (ents.empty() && tails.empty()) continue;
-    const uint32_t n_pass = (uint32_t)ents.size();
+    if (plan.size() == 0 && tails.empty()) continue;
+    std::vector<StageEntry> ents;
+    plan.layout(ents);
+    const uint32_t n_plan = (uint32_t)ents.size();
     ents.insert(ents.end(), tails.begin(), tails.end());
     any = true;
     const int si = take_staging(sv);
@@ -1594,27 +1707,28 @@ int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
     st.busy = true;
     ++staged;
     BatchArgs a;
+    a.tile_base = 0;
+    a.pad0 = 0;
     a.insts = sv->d_insts;
     a.items = sv->items;
     a.partials = sv->d_partials;
     a.fc = ctx->d_fc[fidx];
     const Launchers& ln = launchers_for(fidx);
-    const int tmax = pick_tmax(maxT), pmax = maxP <= 4 ? 4 : 8;
-    if (n_pass) {
-      a.entries = sv->d_entries;
-      a.n_entries = n_pass;
-      SC_HIP(ln.batch_pass(s, tmax, pmax, a, (unsigned)tiles));
-      SC_HIP(ln.batch_finalize(s, pmax, a, n_pass));
-    }
-    if (ents.size() > n_pass) {
-      a.entries = sv->d_entries + n_pass;
-      a.n_entries = (uint32_t)ents.size() - n_pass;
+    const uint32_t n_pass = (uint32_t)plan.pass.size(), n_eval = (uint32_t)plan.eval.size();
+    rc = enqueue_stage(ln, s, a, tmax, pmax, sv->d_entries, n_pass, n_eval, (uint32_t)plan.fold.size(),
+                       plan.pass_tiles, plan.eval_tiles, plan.fold_tiles);
+    if (rc != SC_OK) return rc;
+    if (!tails.empty()) {
+      a.entries = sv->d_entries + n_plan;
+      a.n_entries = (uint32_t)tails.size();
       SC_HIP(ln.batch_tail(s, tmax, pmax, a, a.n_entries));
     }
+    // each request is in exactly one of [pass][eval] or the tails (fold entries repeat eval's)
     for (size_t k = 0; k < ents.size(); ++k) {
+      if (k >= n_pass + n_eval && k < n_plan) continue;
       const StageEntry& e = ents[k];
       Request& r = sv->reqs[sv->slot_req[e.inst]];
-      r.round = k < n_pass ? r.round + 1 : r.num_vars;
+      r.round = k < n_plan ? r.round + 1 : r.num_vars;
       if (r.round >= r.num_vars) {
         SC_HIP(hipMemcpyAsync(r.result, sv->d_states + r.slot, sizeof(ProofState), hipMemcpyDeviceToHost, s));
         r.state = REQ_FINISHING;

@@ -93,6 +93,7 @@ SIGNATURES = [
      [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ScComposition), i32p,
       ctypes.POINTER(ScInput), ctypes.POINTER(ctypes.c_void_p)]),
     ("sc_batch_set_tile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("sc_batch_set_point_parallel", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
     ("sc_batch_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("sc_batch_capture", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("sc_batch_wait", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),

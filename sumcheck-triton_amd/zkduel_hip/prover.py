@@ -387,6 +387,11 @@ class Batch:
     def set_tile(self, items_per_thread: int) -> None:
         _lib.check(self.lib.sc_batch_set_tile(self.handle, int(items_per_thread)))
 
+    def set_point_parallel(self, max_pairs: int = -1) -> None:
+        """Instance rounds with at most ``max_pairs`` pairs run point-parallel (0: never;
+        negative: the default rule)."""
+        _lib.check(self.lib.sc_batch_set_point_parallel(self.handle, int(max_pairs)))
+
     def run(self, challenge_mode: str = "sha3") -> None:
         _lib.check(self.lib.sc_batch_run(self.handle, _lib.CHALLENGE_MODES[challenge_mode]))
 

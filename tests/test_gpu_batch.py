@@ -1,7 +1,7 @@
 """GPU tests of batched proving: every instance of a batch must give the same bits as proving
 it alone (itself checked against the oracle elsewhere), for uneven sizes and mixed
-compositions, any tile size, and hipGraph replays; small instances are also checked against
-the oracle directly."""
+compositions, any tile size, per-pair or point-parallel rounds, and hipGraph replays; small
+instances are also checked against the oracle directly."""
 
 from __future__ import annotations
 
@@ -14,7 +14,7 @@ import pytest
 from oracle import sumcheck_ref as R
 from zkduel_hip import Composition, Context, Proof, get_field, get_spec
 from zkduel_hip.prover import Batch
-from zkduel_hip.synthetic import batch_instances, random_product_sum
+from zkduel_hip.synthetic import batch_instances, constraint_composition, random_product_sum
 
 pytestmark = pytest.mark.gpu
 
@@ -47,18 +47,27 @@ def test_batch_matches_single_proofs(ctx, bw):
                   {"kind": "uniform", "seed": 7, "instance": 100, "selector_mask": 0b10}))
     insts.append((Composition.from_spec(get_spec("poly_abc_plus_de")), 11,
                   {"kind": "affine", "seed": 12, "instance": 0}))
+    if bw in (64, 256):
+        # 10 tables: the batch's default rounds are point-parallel, 0 forces per-pair kernels
+        wide_comp, mask = constraint_composition(bw, tables=8, selectors=2, monomials=6)
+        insts.append((wide_comp, 7, {"kind": "uniform", "seed": 5, "instance": 200, "selector_mask": mask}))
+    wants = {mode: [plain_result(ctx, bw, comp, nv, inp, mode) for comp, nv, inp in insts]
+             for mode in ("sha3", "fixed")}
     with Batch(ctx, bw, insts) as b:
-        for items in (1, 4, 16):
-            b.set_tile(items)
-            for mode in ("sha3", "fixed"):
-                say("batch", bw, items, mode)
-                b.run(mode)
-                b.wait()
-                for i, (comp, nv, inp) in enumerate(insts):
-                    got = b.result(i)
-                    want = plain_result(ctx, bw, comp, nv, inp, mode)
-                    assert np.array_equal(got[0], want[0]), (i, nv)
-                    assert np.array_equal(got[1], want[1]) and got[2] == want[2]
+        # per-pair rounds only, point-parallel rounds at or below 2^7 pairs, all, the default
+        for wide in (0, 1 << 7, 1 << 40, -1):
+            b.set_point_parallel(wide)
+            for items in (1, 4, 16):
+                b.set_tile(items)
+                for mode in ("sha3", "fixed"):
+                    say("batch", bw, wide, items, mode)
+                    b.run(mode)
+                    b.wait()
+                    for i, (comp, nv, inp) in enumerate(insts):
+                        got = b.result(i)
+                        want = wants[mode][i]
+                        assert np.array_equal(got[0], want[0]), (i, nv, wide, items)
+                        assert np.array_equal(got[1], want[1]) and got[2] == want[2]
 
 
 @pytest.mark.parametrize("bw", [64, 256])
@@ -83,6 +92,7 @@ def test_batch_small_instances_match_oracle(ctx, bw):
 def test_batch_graph_replays(ctx):
     insts = batch_instances(n=40, seed=9, nv_lo=4, nv_hi=14)
     with Batch(ctx, 64, insts) as b:
+        b.set_point_parallel(1 << 8)  # both round forms in the captured stages
         b.run("sha3")
         b.wait()
         base = [b.result(i) for i in range(len(insts))]
