@@ -1,9 +1,8 @@
 // SHA3-256 (FIPS 202) for host and device, written from the specification.
 //
-// The transcript hashes at most a few hundred bytes per round, so the permutation is the
-// straightforward 24-round Keccak-f[1600] over 25 64-bit lanes, fully unrolled. On the device
-// it is called with wave-uniform inputs so the compiler can keep the state in scalar
-// registers and run it on the scalar ALU.
+// The transcript hashes at most a few hundred bytes per round. keccak_f1600 is the
+// straightforward 24-round permutation over 25 64-bit words (host code and checks);
+// keccak_f1600_wave is the device form, one state word per lane of a wave.
 #pragma once
 
 #include <stdint.h>
@@ -63,41 +62,77 @@ SC_HD void keccak_f1600(uint64_t A[25]) {
 }
 
 #if defined(__HIPCC__)
-// Keccak-f[1600] spread over one wave: lane x + 5y (< 25) holds A[x + 5y] as two 32-bit
-// halves, and the column sums of theta, the pi permutation and the row terms of chi are lane
-// exchanges. Lanes 25..63 follow along on copies of lane 0..24 data and are ignored.
+// Keccak-f[1600] spread over one wave: lane x + 5y (< 25) holds A[x + 5y]. A round is two
+// exchanges through 25 words of LDS (xs), each one 64-bit store per lane and a few 64-bit
+// loads: theta reads columns x-1 and x+1 of every row (ten words) and forms
+// D[x] = C[x-1] ^ rot(C[x+1], 1) directly; rho, pi and chi then read the theta output at the
+// three lanes whose words pi moves to this lane and to its two row neighbours, each rotated
+// by that source's rho offset. The wave runs in lockstep and LDS serves one wave's accesses
+// in order, so wave-scope fences (which also keep the compiler from reordering the accesses)
+// are the only synchronization. Lanes 25..63 follow along on copies of lane 0..24 data and
+// never store.
 __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
   const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, 64);
   const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, 64);
   return (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ uint64_t rotl64_var(uint64_t x, int s) {
   return s == 0 ? x : ((x << s) | (x >> (64 - s)));
 }
 
-__device__ __forceinline__ void keccak_f1600_wave(uint64_t& a) {
+// The lane whose word pi moves to lane (X, Y): B[y, 2x+3y] = rot(A[x, y]), so lane (X, Y)
+// receives from (x, y) = ((Y - 3X) * 3 mod 5, X).
+__device__ __forceinline__ int keccak_pi_src(int lane) {
+  const int x = lane % 5, y = lane / 5;
+  return ((3 * ((y - 3 * x + 15) % 5)) % 5) + 5 * x;
+}
+
+__device__ __forceinline__ void keccak_f1600_wave(uint64_t& a, uint64_t* xs) {
+  const bool owner = (threadIdx.x & 63) < 25;
   const int lane = (int)(threadIdx.x & 63) % 25;
   const int x = lane % 5, y = lane / 5;
-  // rho offset of this lane's word, and the lane whose word pi moves here: B[y, 2x+3y] =
-  // rot(A[x, y]) means lane (X, Y) receives from (x, y) = ((Y - 3X) * 3 mod 5, X) after rho.
   constexpr int R[25] = {0,  1,  62, 28, 27, 36, 44, 6,  55, 20, 3,  10, 43,
                          25, 39, 41, 45, 15, 21, 8,  18, 2,  61, 56, 14};
-  const int rot = R[lane];
-  const int pi_src = ((3 * ((y - 3 * x + 15) % 5)) % 5) + 5 * x;
-  const int c1 = (x + 4) % 5, c2 = (x + 1) % 5;
-  const int n1 = (x + 1) % 5 + 5 * y, n2 = (x + 2) % 5 + 5 * y;
+  const int cm = (x + 4) % 5, cp = (x + 1) % 5;
+  const int s0 = keccak_pi_src(lane);
+  const int s1 = keccak_pi_src((x + 1) % 5 + 5 * y);
+  const int s2 = keccak_pi_src((x + 2) % 5 + 5 * y);
+  const int r0 = R[s0], r1 = R[s1], r2 = R[s2];
+  // round constant i sits in lane i; lane 0 reads round r's from lane r (a register read,
+  // not a memory load on the round's critical path)
+  const uint64_t rc = keccak_rc((int)(threadIdx.x & 63) < 24 ? (int)(threadIdx.x & 63) : 0);
   for (int round = 0; round < 24; ++round) {
-    // theta: column parity C[x], then D[x] = C[x-1] ^ rot(C[x+1], 1)
-    uint64_t c = a;
+    // theta: a ^= C[x-1] ^ rot(C[x+1], 1), both column parities read in one exchange
+    if (owner) xs[lane] = a;
+    wave_sync_lds();
+    uint64_t c_m = 0, c_p = 0;
 #pragma unroll
-    for (int k = 1; k < 5; ++k) c ^= shfl64(a, x + 5 * ((y + k) % 5));
-    a ^= shfl64(c, c1) ^ rotl64(shfl64(c, c2), 1);
-    // rho and pi
-    const uint64_t b = shfl64(rotl64_var(a, rot), pi_src);
-    // chi and iota
-    a = b ^ (~shfl64(b, n1) & shfl64(b, n2));
-    if (lane == 0 && (threadIdx.x & 63) == 0) a ^= keccak_rc(round);
+    for (int k = 0; k < 5; ++k) {
+      c_m ^= xs[cm + 5 * k];
+      c_p ^= xs[cp + 5 * k];
+    }
+    const uint64_t t = a ^ c_m ^ rotl64(c_p, 1);
+    wave_sync_lds();
+    // rho, pi, chi: this lane's B and its row neighbours' B, read from their pi sources
+    if (owner) xs[lane] = t;
+    wave_sync_lds();
+    const uint64_t b0 = rotl64_var(xs[s0], r0);
+    const uint64_t b1 = rotl64_var(xs[s1], r1);
+    const uint64_t b2 = rotl64_var(xs[s2], r2);
+    wave_sync_lds();
+    a = b0 ^ (~b1 & b2);
+    // (readlane returns int: zero-extend each half, never sign-extend)
+    const uint32_t rc_lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)rc, round);
+    const uint32_t rc_hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(rc >> 32), round);
+    const uint64_t rc_round = (uint64_t)rc_lo | ((uint64_t)rc_hi << 32);
+    if ((threadIdx.x & 63) == 0) a ^= rc_round;  // iota (lane 0 holds A[0, 0])
   }
 }
 #endif

@@ -314,14 +314,16 @@ __device__ __forceinline__ void block_reduce(Elem<F::L> (&acc)[PMAX], int npts, 
   __syncthreads();
 }
 
-// SHA3-256 of a padded message held in LDS, run by one whole wave (lane k of the first 25
-// holds state word k). Returns digest word (lane mod 8) to every lane.
-__device__ __forceinline__ uint32_t sha3_wave_word(const uint32_t* msg, int nblocks) {
+// SHA3-256 of a padded message held in LDS (msg[0 .. 34 * nblocks)), run by one whole wave
+// (lane k of the first 25 holds state word k), exchanging state words through msg[3 * 34 ..
+// MSG_WORDS). Returns digest word (lane mod 8) to every lane.
+__device__ __forceinline__ uint32_t sha3_wave_word(uint32_t* msg, int nblocks) {
   const int lane = threadIdx.x & 63;
+  uint64_t* xs = reinterpret_cast<uint64_t*>(msg + 3 * 34);
   uint64_t a = 0;
   for (int b = 0; b < nblocks; ++b) {
     if (lane < 17) a ^= (uint64_t)msg[34 * b + 2 * lane] | ((uint64_t)msg[34 * b + 2 * lane + 1] << 32);
-    keccak_f1600_wave(a);
+    keccak_f1600_wave(a, xs);
   }
   const uint64_t w = shfl64(a, (lane & 7) >> 1);
   return (lane & 1) ? (uint32_t)(w >> 32) : (uint32_t)w;
@@ -457,7 +459,7 @@ __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
   constexpr int L = F::L;
   __shared__ uint32_t s_red[(BLOCK / 64) * PMAX * L];
   __shared__ uint32_t s_g[PMAX * L];
-  __shared__ uint32_t s_msg[3 * 34];
+  __shared__ __align__(16) uint32_t s_msg[MSG_WORDS];
   __shared__ uint32_t s_r[L];
   __shared__ int s_last;
   const CompDesc* __restrict__ cd = a.comp;
@@ -508,7 +510,7 @@ __global__ void __launch_bounds__(BLOCK) finalize_kernel(FinalizeArgs a) {
   constexpr int L = F::L;
   __shared__ uint32_t s_red[(BLOCK / 64) * PMAX * L];
   __shared__ uint32_t s_g[PMAX * L];
-  __shared__ uint32_t s_msg[3 * 34];
+  __shared__ __align__(16) uint32_t s_msg[MSG_WORDS];
   __shared__ uint32_t s_r[L];
   reduce_partials_and_finish<F, PMAX>(a.partials, a.nblocks, a.comp->npts, a.round, a.num_rounds,
                                       a.challenge_mode, a.fc, a.st, a.explicit_r, s_red, s_g, s_msg, s_r);
@@ -521,7 +523,7 @@ __device__ __forceinline__ void tail_rounds(const TailArgs& a) {
   constexpr int L = F::L;
   __shared__ uint32_t s_red[(TB / 64) * PMAX * L];
   __shared__ uint32_t s_g[PMAX * L];
-  __shared__ uint32_t s_msg[3 * 34];
+  __shared__ __align__(16) uint32_t s_msg[MSG_WORDS];
   __shared__ uint32_t s_r[L];
   const CompDesc* __restrict__ cd = a.comp;
   const int T = Shape<SID>::tables(cd);
@@ -815,7 +817,7 @@ __global__ void __launch_bounds__(BLOCK) batch_finalize_kernel(BatchArgs a) {
   constexpr int L = F::L;
   __shared__ uint32_t s_red[(BLOCK / 64) * PMAX * L];
   __shared__ uint32_t s_g[PMAX * L];
-  __shared__ uint32_t s_msg[3 * 34];
+  __shared__ __align__(16) uint32_t s_msg[MSG_WORDS];
   __shared__ uint32_t s_r[L];
   const StageEntry ent = a.entries[blockIdx.x];
   const InstDesc& in = a.insts[ent.inst];
@@ -932,7 +934,7 @@ __global__ void __launch_bounds__(BLOCK) transcript_kernel(const uint32_t* g, in
                                                           const FieldConsts* fc, ProofState* st) {
   constexpr int L = F::L;
   __shared__ uint32_t s_g[MAX_POINTS * L];
-  __shared__ uint32_t s_msg[3 * 34];
+  __shared__ __align__(16) uint32_t s_msg[MSG_WORDS];
   __shared__ uint32_t s_r[L];
   if (threadIdx.x < npts * L) s_g[threadIdx.x] = g[threadIdx.x];
   __syncthreads();

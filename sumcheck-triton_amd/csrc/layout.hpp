@@ -18,6 +18,9 @@ constexpr int MAX_TERMS = 32;
 constexpr int MAX_POINTS = 8;  // degree <= 7, as cuda/src/field_sumcheck.cu:29
 constexpr int MAX_ROUNDS = 40;
 constexpr int MAX_LIMBS = 8;
+// LDS words for a round's transcript: the padded message (up to 3 SHA3 blocks of 34 words)
+// and the 25 64-bit words the wave Keccak exchanges its state through.
+constexpr int MSG_WORDS = 3 * 34 + 2 * 25;
 constexpr int BLOCK = 256;
 constexpr int MAX_PASS_BLOCKS = 4096;
 
