@@ -60,7 +60,7 @@ def modmuls_per_pair(spec, fold: bool) -> int:
 
 
 def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: float | None,
-                  fixed_schedule: tuple[int, int] | None = None) -> dict:
+                  fixed_schedule: tuple[int, int, int] | None = None) -> dict:
     from zkduel_hip import Composition, Proof, get_field, get_spec
     from zkduel_hip.driver import algorithmic_bytes
 
@@ -73,6 +73,7 @@ def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: floa
     with Proof(ctx, bw, comp, nv, input="uniform", seed=SEED) as pr:
         if fixed_schedule:
             pr.set_schedule(fixed_schedule[0], fixed_schedule[1], False)
+            pr.set_point_parallel(fixed_schedule[2])
             schedule = {"max_pass_blocks": fixed_schedule[0], "tail_pairs": fixed_schedule[1]}
         else:
             schedule = tune_proof(pr)  # kernel choice, grid cap and tail depth timed on this shape
@@ -314,6 +315,9 @@ def main() -> int:
     ap.add_argument("--pass-blocks", type=int, default=0,
                     help="fixed pass-grid cap for the per-width proofs (with --tail-pairs: skip the autotuner)")
     ap.add_argument("--tail-pairs", type=int, default=0)
+    ap.add_argument("--point-parallel", type=int, default=-1,
+                    help="with --pass-blocks/--tail-pairs: rounds with at most this many pairs run "
+                         "point-parallel (-1: the library default)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -331,7 +335,8 @@ def main() -> int:
     for bw in args.widths:
         steps = args.steps if bw == 64 else max(3, args.steps // 2)
         log(f"bench: width {bw} num_vars {SIZES[bw]} steps {steps}")
-        fixed = (args.pass_blocks, args.tail_pairs) if (args.pass_blocks and args.tail_pairs) else None
+        fixed = ((args.pass_blocks, args.tail_pairs, args.point_parallel)
+                 if (args.pass_blocks and args.tail_pairs) else None)
         per[bw] = measure_width(ctx, bw, SIZES[bw], steps, args.warmup, calib["mad_u64_u32"], fixed)
         log(f"bench: width {bw}: {per[bw]['proofs_per_s']:.1f} proofs/s, "
             f"pass kernel {per[bw]['pass_kernel_gbps']:.0f} GB/s")
