@@ -278,6 +278,43 @@ __device__ __forceinline__ void load_pair(typename Work<F>::V (&cur)[TMAX], type
 // Reductions (exact modular sums, so any order gives the same bits)
 // ---------------------------------------------------------------------------------------------
 
+// Cross-lane moves as DPP operand modifiers (register to register, no LDS round trip). CTRL
+// is the gfx9 DPP control: quad_perm, row_half_mirror (lane i <- 7 - i within 8),
+// row_mirror (i <- 15 - i within 16), row_bcast15 / row_bcast31 (lane 15 of a row / lane 31
+// to the following rows selected by ROW_MASK). Lanes in rows outside ROW_MASK read 0.
+constexpr int DPP_QUAD_1032 = 0xB1;  // quad_perm [1, 0, 3, 2]
+constexpr int DPP_QUAD_2301 = 0x4E;  // quad_perm [2, 3, 0, 1]
+constexpr int DPP_ROW_HALF_MIRROR = 0x141;
+constexpr int DPP_ROW_MIRROR = 0x140;
+constexpr int DPP_ROW_BCAST15 = 0x142;
+constexpr int DPP_ROW_BCAST31 = 0x143;
+
+template <int CTRL, int ROW_MASK = 0xF, int L>
+__device__ __forceinline__ Elem<L> dpp_move(const Elem<L>& x) {
+  Elem<L> y;
+#pragma unroll
+  for (int k = 0; k < L; ++k) y.v[k] = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x.v[k], CTRL, ROW_MASK, 0xF, false);
+  return y;
+}
+
+// Sum over each group of 8 lanes (8k .. 8k+7), returned to all lanes of the group.
+template <class F>
+__device__ __forceinline__ Elem<F::L> group8_sum(Elem<F::L> x) {
+  x = add<F>(x, dpp_move<DPP_QUAD_1032>(x));
+  x = add<F>(x, dpp_move<DPP_QUAD_2301>(x));
+  return add<F>(x, dpp_move<DPP_ROW_HALF_MIRROR>(x));
+}
+
+// Sum over the whole wave, valid in lane 63: group sums, row sums (row_mirror), then row r
+// adds row r-1 (bcast15 into rows 1 and 3) and rows 2, 3 add rows 0 + 1 (bcast31).
+template <class F>
+__device__ __forceinline__ Elem<F::L> wave_sum_to_lane63(Elem<F::L> x) {
+  x = group8_sum<F>(x);
+  x = add<F>(x, dpp_move<DPP_ROW_MIRROR>(x));
+  x = add<F>(x, dpp_move<DPP_ROW_BCAST15, 0xA>(x));
+  return add<F>(x, dpp_move<DPP_ROW_BCAST31, 0xC>(x));
+}
+
 // Sums acc over the NT-thread block into out[t*L + limb] (LDS) for t < npts. red needs
 // (NT/64)*PMAX*L words. Ends with a barrier.
 template <class F, int PMAX, int NT>
@@ -289,14 +326,8 @@ __device__ __forceinline__ void block_reduce(Elem<F::L> (&acc)[PMAX], int npts, 
 #pragma unroll
   for (int t = 0; t < PMAX; ++t) {
     if (t < npts) {
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        Elem<L> o;
-#pragma unroll
-        for (int k = 0; k < L; ++k) o.v[k] = __shfl_xor(acc[t].v[k], off, 64);
-        acc[t] = add<F>(acc[t], o);
-      }
-      if (lane == 0) {
+      acc[t] = wave_sum_to_lane63<F>(acc[t]);
+      if (lane == 63) {
 #pragma unroll
         for (int k = 0; k < L; ++k) red[(wave * PMAX + t) * L + k] = acc[t].v[k];
       }
@@ -334,7 +365,7 @@ __device__ __forceinline__ uint32_t sha3_wave_word(uint32_t* msg, int nblocks) {
 // r_out (LDS, L words) and st->r[round]. All threads call it.
 // The claim of round `round` >= 1: g_{round-1}(r_{round-1}) by barycentric interpolation
 // from the previous polynomial's values at 0..npts-1, one point per lane of the calling wave,
-// in the field's working form. Returns the canonical Montgomery value to every lane.
+// in the field's working form. Returns the canonical Montgomery value to lanes 0..7.
 template <class F>
 __device__ __forceinline__ Elem<F::L> previous_claim(const ProofState* st, int round, int npts,
                                                      const FieldConsts* __restrict__ fc) {
@@ -353,14 +384,8 @@ __device__ __forceinline__ Elem<F::L> previous_claim(const ProofState* st, int r
     acc.add(prod);
     term = acc.value();
   }
-#pragma unroll
-  for (int off = 1; off < MAX_POINTS; off <<= 1) {
-    Elem<L> o;
-#pragma unroll
-    for (int k = 0; k < L; ++k) o.v[k] = __shfl_xor(term.v[k], off, 64);
-    term = add<F>(term, o);
-  }
-  return term;
+  static_assert(MAX_POINTS == 8, "one point per lane of an 8-lane group");
+  return group8_sum<F>(term);
 }
 
 // With fill_g1 (every prover path; the per-pair kernels skip point 1 from round 1 on),
@@ -402,13 +427,7 @@ __device__ __forceinline__ void finish_round(uint32_t* g, int npts, int round, i
         w.v[0] = word;
         r = mont_mul<F>(w, load_elem<L>(fc->word_pow[tid]));
       }
-#pragma unroll
-      for (int off = 1; off < 8; off <<= 1) {
-        Elem<L> o;
-#pragma unroll
-        for (int k = 0; k < L; ++k) o.v[k] = __shfl_xor(r.v[k], off, 64);
-        r = add<F>(r, o);
-      }
+      r = group8_sum<F>(r);
       if (tid == 0) {
 #pragma unroll
         for (int k = 0; k < L; ++k) r_out[k] = r.v[k];
