@@ -358,13 +358,20 @@ SC_HD Elem<F::L> mont_mul_fips(const Elem<F::L>& a, const Elem<F::L>& b) {
 // R^-1 = 2^128 = -2^32 and, for x = a*b = x3:x2:x1:x0 (32-bit words),
 //   x * R^-1 = (x3 - x1 - x0) * 2^32 + x1 + x2   (mod p).
 // The 64-bit sums below fold each wrap-around back in with 2^64 = 2^32 - 1 (mod p).
-// Requires a, b < p (any b < 2^64 with a < p keeps x < 2^128).
-SC_HD Elem<2> goldilocks_mont_mul(const Elem<2>& a, const Elem<2>& b) {
+// Any 64-bit operands (x < 2^128 always).
+
+// The 128-bit product a*b as lo (x1:x0) and hi (x3:x2).
+SC_HD uint64_t gl_product(const Elem<2>& a, const Elem<2>& b, uint64_t& hi) {
   const uint64_t p00 = (uint64_t)a.v[0] * b.v[0];
   const uint64_t t0 = (uint64_t)a.v[0] * b.v[1] + (p00 >> 32);
   const uint64_t t1 = (uint64_t)a.v[1] * b.v[0] + (uint32_t)t0;
-  const uint64_t hi = (uint64_t)a.v[1] * b.v[1] + ((t0 >> 32) + (t1 >> 32));
-  const uint64_t x0 = (uint32_t)p00, x1 = (uint32_t)t1;
+  hi = (uint64_t)a.v[1] * b.v[1] + ((t0 >> 32) + (t1 >> 32));
+  return (uint32_t)p00 | ((uint64_t)(uint32_t)t1 << 32);
+}
+
+// x * R^-1 mod p in [0, p) for x = hi:lo < 2^128.
+SC_HD uint64_t gl_reduce(uint64_t lo, uint64_t hi) {
+  const uint64_t x0 = (uint32_t)lo, x1 = lo >> 32;
   // s = x3 * 2^32 + x2 + x1 (hi is x3 * 2^32 + x2)
   const uint64_t add = hi + x1;
   const uint64_t s = add + ((add < hi) ? 0xFFFFFFFFull : 0ull);
@@ -373,11 +380,27 @@ SC_HD Elem<2> goldilocks_mont_mul(const Elem<2>& a, const Elem<2>& b) {
   d -= (d > s) ? 0xFFFFFFFFull : 0ull;
   const uint64_t e = d - (x1 << 32);
   const uint64_t f = e - ((e > d) ? 0xFFFFFFFFull : 0ull);
-  const uint64_t r = f >= 0xFFFFFFFF00000001ull ? f - 0xFFFFFFFF00000001ull : f;
-  Elem<2> out;
-  out.v[0] = (uint32_t)r;
-  out.v[1] = (uint32_t)(r >> 32);
-  return out;
+  return f >= 0xFFFFFFFF00000001ull ? f - 0xFFFFFFFF00000001ull : f;
+}
+
+SC_HD Elem<2> goldilocks_mont_mul(const Elem<2>& a, const Elem<2>& b) {
+  uint64_t hi;
+  const uint64_t lo = gl_product(a, b, hi);
+  return gl_elem(gl_reduce(lo, hi));
+}
+
+// (a1 b1 + a2 b2) R^-1 with one reduction: the two 128-bit products are summed first; a
+// carry out of bit 128 is worth 2^128 R^-1 = 2^64 = 2^32 - 1 (mod p). Any 64-bit operands;
+// returns a value < 2^64 congruent to the result (not reduced below p).
+SC_HD Elem<2> goldilocks_mont_mul_sum(const Elem<2>& a1, const Elem<2>& b1, const Elem<2>& a2, const Elem<2>& b2) {
+  uint64_t h1, h2;
+  const uint64_t l1 = gl_product(a1, b1, h1);
+  const uint64_t l2 = gl_product(a2, b2, h2);
+  const uint64_t lo = l1 + l2;
+  const uint64_t h = h1 + h2;
+  const uint64_t hi = h + (lo < l1 ? 1ull : 0ull);
+  const bool carry = h < h1 || hi < h;
+  return gl_elem(gl_reduce(lo, hi) + (carry ? 0xFFFFFFFFull : 0ull));
 }
 
 // Mersenne p = 2^127 - 1, R = 2^128: 2^127 = 1, so R = 2 and R^-1 = 2^126 (mod p), and

@@ -44,9 +44,13 @@ struct Work {
     for (int i = 0; i < L; ++i) w[i] = x.v[i];
   }
   SC_HD static V mul(const V& a, const V& b) { return mont_mul<F>(a, b); }
-  // a1 b1 + a2 b2
+  // a1 b1 + a2 b2 (Goldilocks: one reduction for both products)
   SC_HD static V mul_sum(const V& a1, const V& b1, const V& a2, const V& b2) {
-    return step_add<F>(mont_mul<F>(a1, b1), mont_mul<F>(a2, b2));
+    if constexpr (F::WIDTH == 64) {
+      return goldilocks_mont_mul_sum(a1, b1, a2, b2);
+    } else {
+      return step_add<F>(mont_mul<F>(a1, b1), mont_mul<F>(a2, b2));
+    }
   }
   SC_HD static V diff(const V& hi, const V& lo) { return sub<F>(hi, lo); }
   SC_HD static V step(const V& x, const V& d) { return step_add<F>(x, d); }

@@ -96,6 +96,15 @@ extern "C" void field_host_work(int bit_width, int op, uint64_t n, const uint32_
   }
 }
 
+// Goldilocks raw words: out[i] = goldilocks_mont_mul_sum(a1, b1, a2, b2) for any 64-bit
+// operands (the unreduced values the kernels pass it), as a 64-bit word.
+extern "C" void gl_host_mul_sum(uint64_t n, const uint64_t* a1, const uint64_t* b1, const uint64_t* a2,
+                                const uint64_t* b2, uint64_t* out) {
+  for (uint64_t i = 0; i < n; ++i) {
+    out[i] = gl_u64(goldilocks_mont_mul_sum(gl_elem(a1[i]), gl_elem(b1[i]), gl_elem(a2[i]), gl_elem(b2[i])));
+  }
+}
+
 // Radix-2^29 helpers of the 255-bit field on raw limbs: op 0 unpack<0> then pack<0> (identity
 // for values < 2^256); op 1 top-limb reduction of the nine limbs in a (limbs < 2^30, top < 2^32).
 extern "C" void r29_host(int op, uint64_t n, const uint32_t* a, uint32_t* out) {

@@ -32,6 +32,9 @@ def harness(tmp_path_factory):
     lib.field_host_work.restype = None
     lib.r29_host.argtypes = [ctypes.c_int, ctypes.c_uint64, u32p, u32p]
     lib.r29_host.restype = None
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    lib.gl_host_mul_sum.argtypes = [ctypes.c_uint64, u64p, u64p, u64p, u64p, u64p]
+    lib.gl_host_mul_sum.restype = None
 
     def op(bw, code, a, b):
         f = get_field(bw)
@@ -54,8 +57,15 @@ def harness(tmp_path_factory):
         lib.r29_host(code, arr.shape[0], arr.ctypes.data_as(u32p), out.ctypes.data_as(u32p))
         return out
 
+    def gl_mul_sum(a1, b1, a2, b2):
+        arrs = [np.ascontiguousarray(x, dtype=np.uint64) for x in (a1, b1, a2, b2)]
+        out = np.zeros_like(arrs[0])
+        lib.gl_host_mul_sum(len(out), *[x.ctypes.data_as(u64p) for x in arrs], out.ctypes.data_as(u64p))
+        return [int(v) for v in out]
+
     op.work = work
     op.r29 = r29
+    op.gl_mul_sum = gl_mul_sum
     return op
 
 
@@ -110,6 +120,28 @@ def test_host_working_form(harness, bw):
         want4.append((8 * mont(x, y) - mont(z, xs % p)) % p)
     assert harness.work(bw, 4, a, b, c) == want4
     assert harness.work(bw, 5, a, b, c) == [(mont(mont(x, t), t) + mont(z, x)) % p for x, t, z in zip(a, x7, c)]
+
+
+def test_host_goldilocks_product_sum_one_reduction(harness):
+    """Goldilocks a1 b1 + a2 b2 with a single reduction, on any 64-bit operands (the kernels
+    pass it values left unreduced below 2^64): the result is below 2^64 and congruent to
+    (a1 b1 + a2 b2) 2^-64 mod p. Operands near 2^64 make the 129-bit sum carry out."""
+    f = get_field(64)
+    p, rinv = f.modulus, f.r_inv
+    rng = random.Random(64064)
+    edge = [0, 1, p - 1, p, p + 1, (1 << 64) - 1, (1 << 64) - 2, (1 << 63), (1 << 32) - 1, 1 << 32]
+    cols = [[], [], [], []]
+    for x in edge:
+        for y in edge:
+            for c, v in zip(cols, (x, y, y, x)):
+                c.append(v)
+    for _ in range(20000):
+        for c in cols:
+            c.append(rng.choice((rng.randrange(1 << 64), (1 << 64) - 1 - rng.randrange(1 << 20),
+                                 rng.randrange(p))))
+    got = harness.gl_mul_sum(*cols)
+    for g, a1, b1, a2, b2 in zip(got, *cols):
+        assert g < (1 << 64) and g % p == (a1 * b1 + a2 * b2) * rinv % p, (a1, b1, a2, b2)
 
 
 def test_host_r29_pack_and_reduce(harness):
