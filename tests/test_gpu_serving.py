@@ -1,6 +1,7 @@
 """GPU tests of the continuous-batching server: every request's result is the same bits alone
-and in the mix (mixed widths, sizes and compositions, requests arriving while others run),
-admission respects the HBM budget, and a Poisson replay completes with identical results."""
+and in the mix (mixed widths, sizes and compositions including requests over 8 tables, whose
+steps run point-parallel, requests arriving while others run), admission respects the HBM
+budget, and a Poisson replay completes with identical results."""
 
 from __future__ import annotations
 
@@ -36,12 +37,25 @@ def alone(ctx, bw, comp, nv, seed, instance, mode="sha3"):
         return pr.result()
 
 
+def wide_composition(rng, tables=10):
+    """More than 8 tables: a step that holds such a request runs its width's rounds
+    point-parallel."""
+    terms = [(1, [rng.randrange(tables) for _ in range(rng.randrange(2, 5))]) for _ in range(4)]
+    terms.append((1, [tables - 1]))
+    return Composition.from_terms(tables, terms)
+
+
 def mixed_requests(n, seed):
     rng = random.Random(seed)
     out = []
     for i in range(n):
         bw = rng.choice((32, 64, 128, 256))
-        comp = random_product_sum(rng) if i % 3 else Composition.from_spec(get_spec("poly_abc_plus_de"))
+        if i % 7 == 5:
+            comp = wide_composition(rng)
+        elif i % 3:
+            comp = random_product_sum(rng)
+        else:
+            comp = Composition.from_spec(get_spec("poly_abc_plus_de"))
         out.append((bw, comp, rng.randrange(1, 13), i))
     return out
 
