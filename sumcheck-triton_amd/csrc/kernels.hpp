@@ -274,6 +274,46 @@ __device__ __forceinline__ void load_pair(typename Work<F>::V (&cur)[TMAX], type
   });
 }
 
+// load_pair with the round's kind fixed at compile time (FOLD: round >= 1; SRC_MONT: src is
+// the caller's Montgomery input), every table's words loaded before any is used. With the
+// kind a run-time value (load_pair) the compiler keeps one branch per table and issues each
+// table's loads after the previous table was folded and stored: one table's loads in flight
+// per thread at a time, but reads and writes interleaved.
+template <class F, int TMAX, bool FOLD, bool SRC_MONT>
+__device__ __forceinline__ void load_pair_fixed(typename Work<F>::V (&cur)[TMAX], typename Work<F>::V (&oth)[TMAX],
+                                                const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                                uint64_t src_n, uint64_t i, int T, const typename Work<F>::V& r) {
+  using W = Work<F>;
+  constexpr int L = F::L;
+  constexpr int NW = FOLD ? 4 * L : 2 * L;
+  uint32_t w[TMAX][NW];
+  table_for<TMAX, (TMAX <= 8)>([&](auto vi) {
+    const int v = vi;
+    if (v < T) load_words<NW>(w[v], src + ((uint64_t)v * src_n + (FOLD ? 4 : 2) * i) * L);
+  });
+  table_for<TMAX, (TMAX <= 8)>([&](auto vi) {
+    const int v = vi;
+    if (v < T) {
+      if constexpr (FOLD) {
+        typename W::V f[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) f[k] = SRC_MONT ? W::from_mont(w[v] + k * L) : W::from_stored(w[v] + k * L);
+        const typename W::V e = W::fold(f[0], f[1], r);
+        const typename W::V o = W::fold(f[2], f[3], r);
+        uint32_t y[2 * L];
+        W::store(y, e);
+        W::store(y + L, o);
+        store_words<2 * L>(dst + ((uint64_t)v * (src_n >> 1) + 2 * i) * L, y);
+        cur[v] = e;
+        oth[v] = o;
+      } else {
+        cur[v] = W::from_mont(w[v]);
+        oth[v] = W::from_mont(w[v] + L);
+      }
+    }
+  });
+}
+
 // ---------------------------------------------------------------------------------------------
 // Reductions (exact modular sums, so any order gives the same bits)
 // ---------------------------------------------------------------------------------------------
@@ -495,10 +535,33 @@ __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
   const bool src_mont = a.round <= 1;
 
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
-  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.pairs; i += stride) {
-    typename W::V cur[TMAX], oth[TMAX];
-    load_pair<F, TMAX>(cur, oth, a.src, a.dst, a.src_n, i, T, fold, src_mont, r);
-    eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts, fold);
+  // The 2^127 - 1 field loads all tables of a pair first, in one loop per round kind
+  // (load_pair_fixed; a*b*c + d*e num_vars 24 on MI355X: 1.69 -> 1.63 ms, same box). The
+  // other fields keep the per-table order: grouped loads measured no faster at 32 bits and
+  // slower at 64 (Goldilocks round 1 798 -> 825 us), and the 255-bit field is bound by its
+  // registers.
+  if constexpr (L == 4) {
+    auto sweep = [&](auto fold_c, auto mont_c) {
+      constexpr bool FOLD = decltype(fold_c)::value, MONT = decltype(mont_c)::value;
+      for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.pairs; i += stride) {
+        typename W::V cur[TMAX], oth[TMAX];
+        load_pair_fixed<F, TMAX, FOLD, MONT>(cur, oth, a.src, a.dst, a.src_n, i, T, r);
+        eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts, FOLD);
+      }
+    };
+    if (!fold) {
+      sweep(std::false_type{}, std::true_type{});
+    } else if (W::kMontIsStored || !src_mont) {
+      sweep(std::true_type{}, std::false_type{});
+    } else {
+      sweep(std::true_type{}, std::true_type{});
+    }
+  } else {
+    for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.pairs; i += stride) {
+      typename W::V cur[TMAX], oth[TMAX];
+      load_pair<F, TMAX>(cur, oth, a.src, a.dst, a.src_n, i, T, fold, src_mont, r);
+      eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts, fold);
+    }
   }
   Elem<L> sums[PMAX];
 #pragma unroll

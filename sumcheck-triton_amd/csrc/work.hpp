@@ -29,6 +29,7 @@ namespace sc {
 template <class F>
 struct Work {
   static constexpr int L = F::L;
+  static constexpr bool kMontIsStored = true;  // from_mont and from_stored read the same form
   using V = Elem<L>;
   SC_HD static V load(const uint32_t* w) {
     V x;
@@ -299,6 +300,7 @@ SC_HD Elem<8> r29_to_mont(const R29& x) {
 template <>
 struct Work<FieldP256> {
   static constexpr int L = 8;
+  static constexpr bool kMontIsStored = false;
   using V = R29;
   SC_HD static V from_mont(const uint32_t* w) { return r29_reduce(r29_unpack<5>(w)); }
   SC_HD static V from_stored(const uint32_t* w) { return r29_unpack<0>(w); }
@@ -487,6 +489,7 @@ SC_HD Elem<4> r26_to_mont(const R26& x) {
 template <>
 struct Work<FieldP128> {
   static constexpr int L = 4;
+  static constexpr bool kMontIsStored = false;
   using V = R26;
   // Montgomery X = 2x (X < p): x is X rotated right by one within 127 bits.
   SC_HD static V from_mont(const uint32_t* w) {
