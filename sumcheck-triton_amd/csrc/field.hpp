@@ -217,11 +217,45 @@ SC_HD uint64_t gl_add_lazy(uint64_t a, uint64_t b) {
   return s < a ? s + GL_EPS : s;
 }
 
-// Canonical add/sub for every field (limb chains; 64-bit words for Goldilocks).
+// ---------------------------------------------------------------------------------------------
+// 2^32 - 5 on one word: 2^32 = 5 (mod p), so subtracting p is adding 5 modulo 2^32.
+// ---------------------------------------------------------------------------------------------
+constexpr uint32_t P32 = 0xFFFFFFFBu;
+
+// a, b < p -> (a + b) mod p: a + b >= p exactly when a + b or a + b + 5 carries out of 32
+// bits (a + b < 2p), and then a + b - p = (a + b + 5) mod 2^32.
+SC_HD uint32_t p32_add(uint32_t a, uint32_t b) {
+  const uint32_t s = a + b;
+  const uint32_t t = s + 5u;
+  return (s < a || t < s) ? t : s;
+}
+// a, b < p -> (a - b) mod p: on a borrow, (a - b) mod 2^32 - 5 = a - b + p (no wrap: >= 1).
+SC_HD uint32_t p32_sub(uint32_t a, uint32_t b) {
+  const uint32_t d = a - b;
+  return a < b ? d - 5u : d;
+}
+// Montgomery product a b 2^-32 mod p in [0, p) for a, b < p: with m = x NPRIME mod 2^32,
+// r = (x + m p) / 2^32 < 2p is the high word h of the 64-bit sum plus its carry c times
+// 2^32; r >= p exactly when c is set or h + 5 carries, and then r - p = (h + 5) mod 2^32.
+SC_HD uint32_t p32_mont_mul(uint32_t a, uint32_t b) {
+  const uint64_t x = (uint64_t)a * b;
+  const uint32_t m = (uint32_t)x * 0xCCCCCCCDu;
+  const uint64_t s = (uint64_t)m * P32 + x;
+  const uint32_t h = (uint32_t)(s >> 32);
+  const uint32_t t = h + 5u;
+  return (s < x || t < h) ? t : h;
+}
+
+// Canonical add/sub for every field (limb chains; 64-bit words for Goldilocks; one word with
+// the 2^32 = 5 identity for 2^32 - 5).
 template <class F>
 SC_HD Elem<F::L> add(const Elem<F::L>& a, const Elem<F::L>& b) {
   if constexpr (F::WIDTH == 64) {
     return gl_elem(gl_add(gl_u64(a), gl_u64(b)));
+  } else if constexpr (F::WIDTH == 32) {
+    Elem<1> s;
+    s.v[0] = p32_add(a.v[0], b.v[0]);
+    return s;
   } else {
     return add_limbs<F>(a, b);
   }
@@ -231,6 +265,10 @@ template <class F>
 SC_HD Elem<F::L> sub(const Elem<F::L>& a, const Elem<F::L>& b) {
   if constexpr (F::WIDTH == 64) {
     return gl_elem(gl_sub(gl_u64(a), gl_u64(b)));
+  } else if constexpr (F::WIDTH == 32) {
+    Elem<1> d;
+    d.v[0] = p32_sub(a.v[0], b.v[0]);
+    return d;
   } else {
     return sub_limbs<F>(a, b);
   }
@@ -267,6 +305,23 @@ struct Accum {
   SC_HD void add(const Elem<F::L>& x) { v = sc::add<F>(v, canon<F>(x)); }
   SC_HD void sub(const Elem<F::L>& x) { v = sc::sub<F>(v, canon<F>(x)); }
   SC_HD Elem<F::L> value() const { return v; }
+};
+
+// 2^32 - 5: a 64-bit sum of canonical words (at most 2^32 of them), reduced once with
+// 2^32 = 5 (mod p): 5 hi + lo < 6 2^32, once more below 2^32 + 30 < 2p, then one subtraction.
+template <>
+struct Accum<FieldP32> {
+  uint64_t s;
+  SC_HD void clear() { s = 0; }
+  SC_HD void add(const Elem<1>& x) { s += x.v[0]; }
+  SC_HD void sub(const Elem<1>& x) { s += P32 - x.v[0]; }  // x < p, so p - x in [1, p]
+  SC_HD Elem<1> value() const {
+    const uint64_t y = (s >> 32) * 5u + (uint32_t)s;
+    const uint64_t z = (y >> 32) * 5u + (uint32_t)y;
+    Elem<1> r;
+    r.v[0] = z >= P32 ? (uint32_t)(z - P32) : (uint32_t)z;
+    return r;
+  }
 };
 
 template <>
@@ -462,6 +517,10 @@ template <class F>
 SC_HD Elem<F::L> mont_mul(const Elem<F::L>& a, const Elem<F::L>& b) {
   if constexpr (F::WIDTH == 64) {
     return goldilocks_mont_mul(a, b);
+  } else if constexpr (F::WIDTH == 32) {
+    Elem<1> r;
+    r.v[0] = p32_mont_mul(a.v[0], b.v[0]);
+    return r;
   } else if constexpr (F::WIDTH == 128) {
     return mersenne127_mont_mul(a, b);
   } else {
