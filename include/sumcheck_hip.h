@@ -96,8 +96,8 @@ int sc_proof_set_schedule(sc_proof* proof, int32_t max_pass_blocks, int64_t tail
 /* Kernel choice per round: rounds with at most max_pairs pairs run as a fold launch plus the
  * point-parallel evaluation (one lane per pair and point) instead of the per-pair pass kernel
  * with the fold fused in; 0 never, a negative value restores the default (every round for
- * interpreted compositions of more than 8 tables, rounds below 2^20 pairs for interpreted
- * 255-bit compositions, none otherwise). Results do not depend on it. No counterpart in the
+ * interpreted compositions of more than 8 tables, rounds below 2^20 pairs for interpreted and
+ * below 2^18 for compiled 255-bit compositions, none otherwise). Results do not depend on it. No counterpart in the
  * reference, which runs every round as an eval, a reduce and a separate fold kernel
  * (field_sumcheck.cu:614-694). */
 int sc_proof_set_point_parallel(sc_proof* proof, int64_t max_pairs);

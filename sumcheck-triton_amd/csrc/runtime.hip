@@ -284,12 +284,20 @@ int default_pass_blocks(int L) {
   }
 }
 
-// Pairs below which the remaining rounds run in the single-workgroup tail kernel. Compositions
-// of more than 8 tables skip it: their per-pair work is large enough that one workgroup is
-// slower than the fold + point-parallel launches even for a handful of pairs.
+// Pairs below which the remaining rounds run in the single-workgroup tail kernel (batch and
+// server instances). Compositions of more than 8 tables skip it: their per-pair work is large
+// enough that one workgroup is slower than the fold + point-parallel launches even for a
+// handful of pairs.
 int64_t default_tail_pairs(int L, bool over8) {
   if (over8) return 1;
   return L == 8 ? 64 : 512;
+}
+
+// The same for a single proof: at 255 bits the tail from 256 pairs measured best (a*b*c + d*e,
+// MI355X: num_vars 24 4.16 -> 4.10 ms, num_vars 20 1.04 -> 1.01 ms against 64 pairs).
+int64_t default_proof_tail_pairs(int L, bool over8) {
+  if (over8) return 1;
+  return L == 8 ? 256 : 512;
 }
 
 // Rounds with at most this many pairs run as a fold launch plus the point-parallel evaluation
@@ -298,10 +306,11 @@ int64_t default_tail_pairs(int L, bool over8) {
 // kernel runs one wave per SIMD (2 x 8 working values and 8 accumulators of 9 limbs), which
 // the point-parallel split beats by 8-30% at num_vars 16-20 (MI355X); above that the per-pair
 // kernel's fused fold saves a write and a read of every table and wins for most compositions.
-// Compiled compositions keep the per-pair kernels (a*b*c + d*e, 255-bit, num_vars 24:
-// 4.3 ms against 6.2 ms all point-parallel).
+// Compiled compositions keep the per-pair kernels for their large rounds (a*b*c + d*e, 255-bit,
+// num_vars 24: 4.3 ms against 6.2 ms all point-parallel); at 255 bits their rounds below 2^18
+// pairs run point-parallel (num_vars 20: 1.01 -> 0.97 ms; num_vars 24: 4.10 -> 4.08 ms).
 int64_t default_wide_pairs(const sc_proof* p) {
-  if (p->sid != 0) return 0;
+  if (p->sid != 0) return p->L == 8 ? (int64_t(1) << 17) : 0;
   if (p->tmax > 8) return INT64_MAX;
   return p->L == 8 ? (int64_t(1) << 19) : 0;
 }
@@ -613,7 +622,7 @@ int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* co
   p->tmax = pick_tmax(p->T);
   p->pmax = p->npts <= 4 ? 4 : 8;
   p->n0 = (uint64_t)1 << num_vars;
-  p->tail_pairs = default_tail_pairs(p->L, p->tmax > 8);
+  p->tail_pairs = default_proof_tail_pairs(p->L, p->tmax > 8);
   p->max_pass_blocks = default_pass_blocks(p->L);
   p->matched_sid = match_static(desc);
   p->sid = p->matched_sid;
@@ -702,7 +711,7 @@ int sc_proof_set_schedule(sc_proof* p, int32_t max_pass_blocks, int64_t tail_pai
   if (tail_pairs < 0) return fail(SC_ERR_ARG, "tail_pairs must be >= 0");
   p->fused_finalize = fused_finalize != 0;
   p->max_pass_blocks = max_pass_blocks ? max_pass_blocks : default_pass_blocks(p->L);
-  p->tail_pairs = tail_pairs ? tail_pairs : default_tail_pairs(p->L, p->tmax > 8);
+  p->tail_pairs = tail_pairs ? tail_pairs : default_proof_tail_pairs(p->L, p->tmax > 8);
   drop_graph(p);
   return SC_OK;
 }
