@@ -9,7 +9,9 @@ Workload (one step = one whole proof, results copied to the host):
 
 Prints ONE JSON line. `roofline` is for the dominant kernel (the fused fold+eval pass
 kernel): algorithmic bytes of every pass launch in the timed region divided by their summed
-durations, both measured here with hipEvents recorded on the stream the kernels run on.
+durations, both measured here with hipEvents recorded on the stream the kernels run on. The
+timed steps run eagerly (per-round event records need that); each width also reports the same
+proof replayed as one captured hipGraph (`graph_ms_per_proof`), timed separately.
 `cpu_baseline` times the oracle's single-core Python restatement of the reference prover
 (src/zkduel/reference.py:44-110) on a bounded sample and scales it to the full size.
 
@@ -79,6 +81,10 @@ def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: floa
             schedule = tune_proof(pr)  # kernel choice, grid cap and tail depth timed on this shape
         # rounds at or below this pair count run fold + point-parallel evaluation, not pass_kernel
         wide_pairs = pr.info()["point_parallel_pairs"]
+        schedule = {"max_pass_blocks": schedule["max_pass_blocks"], "tail_pairs": schedule["tail_pairs"],
+                    "point_parallel_pairs": wide_pairs,
+                    "library_default": bool(schedule.get("library_default", False)),
+                    "tuned_ms": schedule.get("ms"), "default_ms": schedule.get("default_ms")}
         pr.set_profiling(True)
         for _ in range(warmup):
             pr.run("sha3")
@@ -105,6 +111,19 @@ def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: floa
         ctx.sync()
         wall = time.perf_counter() - t0
         info = pr.info()
+        # the same proof replayed as one hipGraph (one host launch instead of one per kernel)
+        pr.set_profiling(False)
+        pr.capture("sha3")
+        pr.run("sha3")
+        pr.wait()
+        ctx.sync()
+        g0 = time.perf_counter()
+        for _ in range(steps):
+            pr.run("sha3")
+            pr.wait()
+            pr.result()
+        ctx.sync()
+        graph_ms = (time.perf_counter() - g0) * 1000.0 / steps
     lat.sort()
     achieved = pass_bytes_sum / (pass_ms_sum * 1e6) if pass_ms_sum > 0 else 0.0
     algo = algorithmic_bytes(T, f.limbs, nv)
@@ -126,17 +145,19 @@ def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: floa
         "hbm_frac": achieved / HBM_PEAK_GBPS,
         "pass_limb_macs_per_s": pass_macs_sum / (pass_ms_sum * 1e-3) if pass_ms_sum > 0 else 0.0,
         "int32_frac": (pass_macs_sum / (pass_ms_sum * 1e-3)) / mad_rate if (mad_rate and pass_ms_sum > 0) else None,
-        "schedule": {"max_pass_blocks": schedule["max_pass_blocks"], "tail_pairs": schedule["tail_pairs"],
-                     "point_parallel_pairs": wide_pairs},
+        "schedule": schedule,
+        "graph_ms_per_proof": graph_ms,
         "wall_s": wall,
     }
 
 
 def measure_plain(ctx, bw: int, comp, nv: int, reps: int, selector_mask: int = 0) -> dict:
-    """Median whole-proof time of one prepared instance (SHA3, uniform tables)."""
+    """Median whole-proof time of one prepared instance (SHA3, uniform tables), each proof
+    replayed as one captured hipGraph."""
     from zkduel_hip import Proof
 
     with Proof(ctx, bw, comp, nv, input="uniform", seed=SEED, selector_mask=selector_mask) as pr:
+        pr.capture("sha3")
         pr.run("sha3")
         pr.wait()
         lat = []

@@ -10,6 +10,10 @@ Knobs, none of which changes any result bit:
   point-parallel evaluation instead of the per-pair pass kernel (-1: the library's default);
 * ``items`` (batches) — pairs per thread in a batch tile.
 
+The library's default schedule (0 / 0 / -1) is always a candidate and is kept unless the
+best tuned schedule beats it by more than ``margin`` (2% by default): candidates closer than
+that differ by timing noise, and a noisy pick would change the schedule from run to run.
+
 Results are cached per (bit_width, composition, num_vars, compiled or interpreted kernels) for
 the process, and, when ``path`` is given, in a JSON file, so a serving process can tune once
 per shape class.
@@ -20,6 +24,7 @@ from __future__ import annotations
 import hashlib
 import json
 import os
+import time
 from pathlib import Path
 from typing import Iterable
 
@@ -73,10 +78,11 @@ def _apply(proof: Proof, choice: dict) -> None:
 def tune_proof(proof: Proof, challenge_mode: str = "sha3", repeats: int = 3,
                pass_blocks: Iterable[int] = PASS_BLOCKS, tail_pairs: Iterable[int] = TAIL_PAIRS,
                wide_pairs: Iterable[int] = WIDE_PAIRS,
-               path: str | os.PathLike | None = None) -> dict:
+               path: str | os.PathLike | None = None, margin: float = 0.02) -> dict:
     """Sets the fastest (max_pass_blocks, tail_pairs, wide_pairs) on ``proof`` and returns
-    {"max_pass_blocks", "tail_pairs", "wide_pairs", "ms", "candidates": {"blocks/tail/wide":
-    ms}}. A cached choice for the same shape is applied without timing."""
+    {"max_pass_blocks", "tail_pairs", "wide_pairs", "ms", "default_ms", "library_default",
+    "candidates": {"blocks/tail/wide": ms}}; 0 / 0 / -1 with ``library_default`` True when the
+    default schedule is kept. A cached choice for the same shape is applied without timing."""
     p = Path(path) if path is not None else None
     _load(p)
     key = _key(proof)
@@ -93,13 +99,32 @@ def tune_proof(proof: Proof, challenge_mode: str = "sha3", repeats: int = 3,
             timings[name] = _time(proof, challenge_mode, repeats)
         return timings[name]
 
+    # the device runs slow for the first few tens of milliseconds of work after idling; without
+    # this the first candidate timed (the default) reads several percent slow
+    _apply(proof, {"max_pass_blocks": 0, "tail_pairs": 0, "wide_pairs": -1})
+    t0 = time.perf_counter()
+    runs = 0
+    while runs < 3 or time.perf_counter() - t0 < 0.1:
+        proof.run(challenge_mode)
+        proof.wait()
+        runs += 1
+    default_ms = timed(0, 0, -1)
     # the knobs are nearly independent: sweep the kernel choice at the default grid and tail,
     # then the grid cap, then the tail, each at the best of the ones before
     w_best = min(wide_pairs, key=lambda w: timed(0, 0, w))
     b_best = min(pass_blocks, key=lambda b: timed(b, 0, w_best))
     t_best = min(tail_pairs, key=lambda t: timed(b_best, t, w_best))
+    best_ms = timings[f"{b_best}/{t_best}/{w_best}"]
+    # timed again last, so the default is not judged on its first (earliest) timing alone
+    _apply(proof, {"max_pass_blocks": 0, "tail_pairs": 0, "wide_pairs": -1})
+    default_ms = min(default_ms, _time(proof, challenge_mode, repeats))
+    timings["0/0/-1"] = default_ms
+    keep_default = best_ms >= default_ms * (1.0 - margin)
+    if keep_default:
+        b_best, t_best, w_best, best_ms = 0, 0, -1, default_ms
     choice = {"max_pass_blocks": b_best, "tail_pairs": t_best, "wide_pairs": w_best,
-              "ms": timings[f"{b_best}/{t_best}/{w_best}"], "candidates": timings}
+              "ms": best_ms, "default_ms": default_ms, "library_default": keep_default,
+              "candidates": timings}
     _apply(proof, choice)
     _cache[key] = choice
     _store(p)
