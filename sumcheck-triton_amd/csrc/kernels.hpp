@@ -386,17 +386,20 @@ __device__ __forceinline__ void block_reduce(Elem<F::L> (&acc)[PMAX], int npts, 
 }
 
 // SHA3-256 of a padded message held in LDS (msg[0 .. 34 * nblocks)), run by one whole wave
-// (lane k of the first 25 holds state word k), exchanging state words through msg[3 * 34 ..
-// MSG_WORDS). Returns digest word (lane mod 8) to every lane.
+// (lanes h * 32 + k hold half h of state word k, interleaved form), exchanging state halves
+// through msg[3 * 34 .. MSG_WORDS). Returns digest word (lane mod 8) to every lane.
 __device__ __forceinline__ uint32_t sha3_wave_word(uint32_t* msg, int nblocks) {
   const int lane = threadIdx.x & 63;
-  uint64_t* xs = reinterpret_cast<uint64_t*>(msg + 3 * 34);
-  uint64_t a = 0;
+  const int h = keccak_half(), p = keccak_word();
+  uint32_t* xs = msg + 3 * 34;
+  uint32_t a = 0;
   for (int b = 0; b < nblocks; ++b) {
-    if (lane < 17) a ^= (uint64_t)msg[34 * b + 2 * lane] | ((uint64_t)msg[34 * b + 2 * lane + 1] << 32);
+    if (p < 17) a ^= interleaved_half((uint64_t)msg[34 * b + 2 * p] | ((uint64_t)msg[34 * b + 2 * p + 1] << 32), h);
     keccak_f1600_wave(a, xs);
   }
-  const uint64_t w = shfl64(a, (lane & 7) >> 1);
+  const int i = (lane & 7) >> 1;  // digest words 2i, 2i + 1 are state word i
+  const uint32_t e = (uint32_t)__shfl((int)a, i, 64), o = (uint32_t)__shfl((int)a, 32 + i, 64);
+  const uint64_t w = interleaved_join(e, o);
   return (lane & 1) ? (uint32_t)(w >> 32) : (uint32_t)w;
 }
 

@@ -3,6 +3,7 @@
 #include <stdint.h>
 
 #include "field.hpp"
+#include "keccak.hpp"
 #include "work.hpp"
 
 using namespace sc;
@@ -117,5 +118,14 @@ extern "C" void r29_host(int op, uint64_t n, const uint32_t* a, uint32_t* out) {
       const R29 r = r29_reduce(x);
       for (int k = 0; k < 9; ++k) out[i * 9 + k] = r.l[k];
     }
+  }
+}
+
+// Bit interleaving of the wave Keccak (csrc/keccak.hpp): the halves of each word and the join.
+extern "C" void interleave_host(uint64_t n, const uint64_t* w, uint32_t* even, uint32_t* odd, uint64_t* joined) {
+  for (uint64_t i = 0; i < n; ++i) {
+    even[i] = interleaved_half(w[i], 0);
+    odd[i] = interleaved_half(w[i], 1);
+    joined[i] = interleaved_join(even[i], odd[i]);
   }
 }

@@ -35,6 +35,8 @@ def harness(tmp_path_factory):
     u64p = ctypes.POINTER(ctypes.c_uint64)
     lib.gl_host_mul_sum.argtypes = [ctypes.c_uint64, u64p, u64p, u64p, u64p, u64p]
     lib.gl_host_mul_sum.restype = None
+    lib.interleave_host.argtypes = [ctypes.c_uint64, u64p, u32p, u32p, u64p]
+    lib.interleave_host.restype = None
 
     def op(bw, code, a, b):
         f = get_field(bw)
@@ -65,7 +67,16 @@ def harness(tmp_path_factory):
 
     op.work = work
     op.r29 = r29
+    def interleave(words):
+        w = np.ascontiguousarray(words, dtype=np.uint64)
+        e, o = np.zeros(len(w), np.uint32), np.zeros(len(w), np.uint32)
+        back = np.zeros_like(w)
+        lib.interleave_host(len(w), w.ctypes.data_as(u64p), e.ctypes.data_as(u32p), o.ctypes.data_as(u32p),
+                            back.ctypes.data_as(u64p))
+        return e, o, back
+
     op.gl_mul_sum = gl_mul_sum
+    op.interleave = interleave
     return op
 
 
@@ -162,3 +173,32 @@ def test_host_r29_pack_and_reduce(harness):
         v = sum(int(x) << (29 * i) for i, x in enumerate(row_in))
         r = sum(int(x) << (29 * i) for i, x in enumerate(row_out))
         assert r % p == v % p and r < p + (1 << 241) and all(x < (1 << 29) for x in row_out)
+
+
+def test_host_keccak_bit_interleaving(harness):
+    """The wave Keccak holds each 64-bit word as its even bits and its odd bits: check both
+    halves bit by bit against the definition, the join back, and that a 64-bit rotation by r
+    is the two 32-bit rotations the kernel uses (r even: both halves by r/2; r odd: the even
+    half takes the odd half rotated by (r+1)/2, the odd half the even half by (r-1)/2)."""
+    rng = random.Random(25)
+    words = [0, 1, 1 << 63, (1 << 64) - 1, 0x5555555555555555, 0xAAAAAAAAAAAAAAAA] + \
+        [rng.getrandbits(64) for _ in range(2000)]
+    e, o, back = harness.interleave(words)
+    for w, ev, od, bk in zip(words, e.tolist(), o.tolist(), back.tolist()):
+        assert ev == sum(((w >> (2 * i)) & 1) << i for i in range(32))
+        assert od == sum(((w >> (2 * i + 1)) & 1) << i for i in range(32))
+        assert bk == w
+
+    def rotl(x, r, bits):
+        r %= bits
+        return ((x << r) | (x >> (bits - r))) & ((1 << bits) - 1) if r else x
+
+    for r in range(64):
+        rotated = [rotl(w, r, 64) for w in words[:200]]
+        re, ro, _ = harness.interleave(rotated)
+        ev, od = e[:200].tolist(), o[:200].tolist()
+        if r % 2 == 0:
+            want_e, want_o = [rotl(x, r // 2, 32) for x in ev], [rotl(x, r // 2, 32) for x in od]
+        else:
+            want_e, want_o = [rotl(x, (r + 1) // 2, 32) for x in od], [rotl(x, (r - 1) // 2, 32) for x in ev]
+        assert re.tolist() == want_e and ro.tolist() == want_o, r
