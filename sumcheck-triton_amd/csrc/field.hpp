@@ -324,27 +324,45 @@ struct Accum<FieldP32> {
   }
 };
 
+SC_HD uint64_t gl_product(const Elem<2>& a, const Elem<2>& b, uint64_t& hi);
+SC_HD uint64_t gl_reduce(uint64_t lo, uint64_t hi);
+
+// Goldilocks: the sum is held as an integer top 2^128 + W (W = w_hi 2^64 + w_lo) whose
+// value is (top 2^128 + W) R^-1 mod p, R = 2^64. A Montgomery value X enters as X 2^64 (an
+// add to w_hi); a product A B of two Montgomery values enters as the raw 128-bit product, so
+// its reduction by R^-1 happens once for the whole sum, in value(), instead of once per
+// product. 2^128 R^-1 = 2^64 = 2^32 - 1 (mod p) prices each carry out of bit 127.
 template <>
 struct Accum<FieldP64> {
-  uint64_t lo;
-  uint32_t carries;
+  uint64_t w_lo, w_hi;
+  uint32_t top;
   SC_HD void clear() {
-    lo = 0;
-    carries = 0;
+    w_lo = 0;
+    w_hi = 0;
+    top = 0;
   }
-  SC_HD void add(const Elem<2>& x) {  // any x < 2^64
-    const uint64_t v = gl_u64(x);
-    lo += v;
-    carries += lo < v ? 1u : 0u;
+  SC_HD void add_hi(uint64_t v) {
+    const uint64_t s = w_hi + v;
+    top += s < v ? 1u : 0u;
+    w_hi = s;
   }
+  SC_HD void add(const Elem<2>& x) { add_hi(gl_u64(x)); }  // any x < 2^64
   SC_HD void sub(const Elem<2>& x) {  // adds p - canon(x) (0 stays 0)
     const uint64_t c = gl_canon(gl_u64(x));
-    add(gl_elem(c == 0 ? 0 : GL_P - c));
+    add_hi(c == 0 ? 0 : GL_P - c);
+  }
+  // W += a b for any a, b < 2^64 (the product's high word is at most 2^64 - 2, so it absorbs
+  // the carry of the low-word add)
+  SC_HD void add_product(const Elem<2>& a, const Elem<2>& b) {
+    uint64_t hi;
+    const uint64_t lo = gl_product(a, b, hi);
+    const uint64_t s = w_lo + lo;
+    w_lo = s;
+    add_hi(hi + (s < lo ? 1u : 0u));
   }
   SC_HD Elem<2> value() const {
-    // lo + carries * (2^32 - 1), carries < 2^32
-    const uint64_t k = (uint64_t)carries * GL_EPS;
-    return gl_elem(gl_canon(gl_add_lazy(gl_canon(lo), k)));
+    const uint64_t r = gl_reduce(w_lo, w_hi);  // W R^-1 mod p
+    return gl_elem(gl_add(r, gl_canon((uint64_t)top * GL_EPS)));
   }
 };
 

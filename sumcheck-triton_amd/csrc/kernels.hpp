@@ -139,6 +139,18 @@ __device__ __forceinline__ void compose_add(typename Work<F>::Acc& acc, const ty
   }
 }
 
+// acc += a b. Goldilocks adds the raw 128-bit product to its wide accumulator (one reduction
+// per thread for the whole sum instead of one per product).
+template <class F>
+__device__ __forceinline__ void acc_add_mul(typename Work<F>::Acc& acc, const typename Work<F>::V& a,
+                                            const typename Work<F>::V& b) {
+  if constexpr (std::is_same_v<F, FieldP64>) {
+    acc.add_product(a, b);
+  } else {
+    acc.add(Work<F>::mul(a, b));
+  }
+}
+
 // Compiled composition STATIC_COMPS[SID]: every loop has constant bounds and indices, so it
 // unrolls into straight-line field operations on fixed registers.
 template <class F, int SID, int TMAX>
@@ -159,23 +171,32 @@ __device__ __forceinline__ void compose_static(typename Work<F>::Acc& acc, const
         pre[k] = W::mul(pre[k], x[STATIC_COMPS[SID].vars[k][m]]);
       });
     });
-    acc.add(W::mul_sum(pre[0], x[C2.vars[0][C2.lens[0] - 1]], pre[1], x[C2.vars[1][C2.lens[1] - 1]]));
+    if constexpr (std::is_same_v<F, FieldP64>) {
+      acc.add_product(pre[0], x[C2.vars[0][C2.lens[0] - 1]]);
+      acc.add_product(pre[1], x[C2.vars[1][C2.lens[1] - 1]]);
+    } else {
+      acc.add(W::mul_sum(pre[0], x[C2.vars[0][C2.lens[0] - 1]], pre[1], x[C2.vars[1][C2.lens[1] - 1]]));
+    }
     return;
   }
   static_for<STATIC_COMPS[SID].num_terms>([&](auto ki) {
     constexpr int k = decltype(ki)::value;
     constexpr StaticComp C = STATIC_COMPS[SID];
     typename W::V prod = x[C.vars[k][0]];
-    static_for<C.lens[k] - 1>([&](auto mi) {
+    // a unit-coefficient product keeps its last multiplication for the accumulator
+    constexpr int kept = (C.kinds[k] == COEFF_ONE && C.lens[k] >= 2) ? 1 : 0;
+    static_for<C.lens[k] - 1 - kept>([&](auto mi) {
       constexpr int m = decltype(mi)::value + 1;
       prod = W::mul(prod, x[STATIC_COMPS[SID].vars[k][m]]);
     });
-    if constexpr (C.kinds[k] == COEFF_ONE) {
+    if constexpr (kept == 1) {
+      acc_add_mul<F>(acc, prod, x[C.vars[k][C.lens[k] - 1]]);
+    } else if constexpr (C.kinds[k] == COEFF_ONE) {
       acc.add(prod);
     } else if constexpr (C.kinds[k] == COEFF_MINUS_ONE) {
       acc.sub(prod);
     } else {
-      acc.add(W::mul(prod, W::coeff(cd->coeff[k])));
+      acc_add_mul<F>(acc, prod, W::coeff(cd->coeff[k]));
     }
   });
 }
@@ -543,7 +564,25 @@ __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
   // other fields keep the per-table order: grouped loads measured no faster at 32 bits and
   // slower at 64 (Goldilocks round 1 798 -> 825 us), and the 255-bit field is bound by its
   // registers.
-  if constexpr (L == 4) {
+  if (L <= 4 && !fold) {
+    // Round 0 only reads: the next pair's loads are issued before this pair is evaluated,
+    // so a wave's evaluation overlaps its own next memory round trip (Goldilocks num_vars 26
+    // proof 2.48 -> 2.45 ms, 2^32-5 1.38 -> 1.35 ms). The 255-bit field has no registers
+    // to spare for it.
+    uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    typename W::V cur[TMAX], oth[TMAX];
+    if (i < a.pairs) load_pair_fixed<F, TMAX, false, true>(cur, oth, a.src, a.dst, a.src_n, i, T, r);
+    for (; i < a.pairs; i += stride) {
+      typename W::V ncur[TMAX], noth[TMAX];
+      if (i + stride < a.pairs) load_pair_fixed<F, TMAX, false, true>(ncur, noth, a.src, a.dst, a.src_n, i + stride, T, r);
+      eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts, false);
+#pragma unroll
+      for (int v = 0; v < TMAX; ++v) {
+        cur[v] = ncur[v];
+        oth[v] = noth[v];
+      }
+    }
+  } else if constexpr (L == 4) {
     auto sweep = [&](auto fold_c, auto mont_c) {
       constexpr bool FOLD = decltype(fold_c)::value, MONT = decltype(mont_c)::value;
       for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.pairs; i += stride) {
@@ -552,9 +591,7 @@ __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
         eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts, FOLD);
       }
     };
-    if (!fold) {
-      sweep(std::false_type{}, std::true_type{});
-    } else if (W::kMontIsStored || !src_mont) {
+    if (W::kMontIsStored || !src_mont) {
       sweep(std::true_type{}, std::false_type{});
     } else {
       sweep(std::true_type{}, std::true_type{});

@@ -129,3 +129,15 @@ extern "C" void interleave_host(uint64_t n, const uint64_t* w, uint32_t* even, u
     joined[i] = interleaved_join(even[i], odd[i]);
   }
 }
+
+// Goldilocks running sum (Accum<FieldP64>): op 0 adds a, 1 subtracts a, 2 adds the product a b.
+extern "C" uint64_t gl_host_accum(uint64_t n, const int32_t* ops, const uint64_t* a, const uint64_t* b) {
+  Accum<FieldP64> acc;
+  acc.clear();
+  for (uint64_t i = 0; i < n; ++i) {
+    if (ops[i] == 0) acc.add(gl_elem(a[i]));
+    else if (ops[i] == 1) acc.sub(gl_elem(a[i]));
+    else acc.add_product(gl_elem(a[i]), gl_elem(b[i]));
+  }
+  return gl_u64(acc.value());
+}

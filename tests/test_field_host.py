@@ -37,6 +37,8 @@ def harness(tmp_path_factory):
     lib.gl_host_mul_sum.restype = None
     lib.interleave_host.argtypes = [ctypes.c_uint64, u64p, u32p, u32p, u64p]
     lib.interleave_host.restype = None
+    lib.gl_host_accum.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32), u64p, u64p]
+    lib.gl_host_accum.restype = ctypes.c_uint64
 
     def op(bw, code, a, b):
         f = get_field(bw)
@@ -75,7 +77,14 @@ def harness(tmp_path_factory):
                             back.ctypes.data_as(u64p))
         return e, o, back
 
+    def gl_accum(ops, a, b):
+        o = np.ascontiguousarray(ops, dtype=np.int32)
+        A, B = (np.ascontiguousarray(x, dtype=np.uint64) for x in (a, b))
+        return int(lib.gl_host_accum(len(o), o.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                     A.ctypes.data_as(u64p), B.ctypes.data_as(u64p)))
+
     op.gl_mul_sum = gl_mul_sum
+    op.gl_accum = gl_accum
     op.interleave = interleave
     return op
 
@@ -202,3 +211,31 @@ def test_host_keccak_bit_interleaving(harness):
         else:
             want_e, want_o = [rotl(x, (r + 1) // 2, 32) for x in od], [rotl(x, (r - 1) // 2, 32) for x in ev]
         assert re.tolist() == want_e and ro.tolist() == want_o, r
+
+
+def test_host_goldilocks_wide_accumulator(harness):
+    """The Goldilocks running sum holds W = top 2^128 + w_hi 2^64 + w_lo with value W R^-1:
+    values enter as X 2^64, products A B raw, and one reduction at the end. Checked against
+    Python integers on long runs of near-2^64 operands, where every carry out of bit 63 and
+    of bit 127 happens many times."""
+    p = (1 << 64) - (1 << 32) + 1
+    rinv = pow(1 << 64, -1, p)
+    rng = random.Random(64)
+    for case in range(6):
+        n = 3000 if case < 5 else 20000
+        hi_bias = case % 2 == 0
+        ops = [rng.choice((0, 1, 2, 2)) for _ in range(n)]
+        if case == 4:
+            ops = [2] * n  # products only: bit 127 carries on every few adds
+        draw = (lambda: (1 << 64) - 1 - rng.randrange(1 << 20)) if hi_bias else (lambda: rng.getrandbits(64))
+        a = [draw() for _ in range(n)]
+        b = [draw() for _ in range(n)]
+        want = 0
+        for o, x, y in zip(ops, a, b):
+            if o == 0:
+                want += x
+            elif o == 1:
+                want -= x
+            else:
+                want += x * y * rinv
+        assert harness.gl_accum(ops, a, b) == want % p, case
