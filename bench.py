@@ -182,7 +182,7 @@ def measure_batched(ctx, bw: int, n: int = 256, reps: int = 3) -> dict:
 
     insts = batch_instances(n, seed=4)
     with Batch(ctx, bw, insts) as b:
-        tune = b.autotune((1, 2, 4, 8), "sha3")
+        tune = b.autotune((1, 2, 4, 8, 16, 32), "sha3")
         best = min(tune, key=tune.get)
         b.capture("sha3")
         b.run("sha3")

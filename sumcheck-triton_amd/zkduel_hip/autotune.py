@@ -131,7 +131,7 @@ def tune_proof(proof: Proof, challenge_mode: str = "sha3", repeats: int = 3,
     return choice
 
 
-def tune_batch(batch: Batch, challenge_mode: str = "sha3", items: Iterable[int] = (1, 2, 4, 8),
+def tune_batch(batch: Batch, challenge_mode: str = "sha3", items: Iterable[int] = (1, 2, 4, 8, 16, 32),
                repeats: int = 1) -> dict[int, float]:
     """Tile size of a batch (pairs per thread), chosen by timing the actual mix."""
     return batch.autotune(tuple(items), challenge_mode, repeats)

@@ -346,7 +346,7 @@ class Batch:
                                             nvs.ctypes.data_as(_lib.i32p), inps, ctypes.byref(handle)))
         self.handle = handle
 
-    def autotune(self, candidates: Sequence[int] = (1, 2, 4, 8), challenge_mode: str = "sha3",
+    def autotune(self, candidates: Sequence[int] = (1, 2, 4, 8, 16, 32), challenge_mode: str = "sha3",
                  repeats: int = 1) -> dict[int, float]:
         """Times one batch run per tile size (pairs per thread) after a warm-up run and keeps
         the fastest; returns {items: ms}. The tile changes only the schedule, never the
