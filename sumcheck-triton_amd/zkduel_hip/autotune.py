@@ -115,7 +115,7 @@ def tune_proof(proof: Proof, challenge_mode: str = "sha3", repeats: int = 3,
     b_best = min(pass_blocks, key=lambda b: timed(b, 0, w_best))
     t_best = min(tail_pairs, key=lambda t: timed(b_best, t, w_best))
     best_ms = timings[f"{b_best}/{t_best}/{w_best}"]
-    # timed again last, so the default is not judged on its first (earliest) timing alone
+    # timed again last, so the default is not compared on its first (earliest) timing alone
     _apply(proof, {"max_pass_blocks": 0, "tail_pairs": 0, "wide_pairs": -1})
     default_ms = min(default_ms, _time(proof, challenge_mode, repeats))
     timings["0/0/-1"] = default_ms
