@@ -279,7 +279,7 @@ int default_pass_blocks(int L) {
   switch (L) {
     case 1: return 1024;  // measured ~10% faster than 2048 at num_vars 26
     case 2: return 1024;
-    case 4: return 2048;
+    case 4: return 1024;  // the autotuner's pick at num_vars 24 (512-1024 beat 2048 by 2-3%)
     default: return 512;
   }
 }
@@ -294,10 +294,11 @@ int64_t default_tail_pairs(int L, bool over8) {
 }
 
 // The same for a single proof: at 255 bits the tail from 256 pairs measured best (a*b*c + d*e,
-// MI355X: num_vars 24 4.16 -> 4.10 ms, num_vars 20 1.04 -> 1.01 ms against 64 pairs).
+// MI355X: num_vars 24 4.16 -> 4.10 ms, num_vars 20 1.04 -> 1.01 ms against 64 pairs), and at
+// 127 bits too (the autotuner's pick at num_vars 24).
 int64_t default_proof_tail_pairs(int L, bool over8) {
   if (over8) return 1;
-  return L == 8 ? 256 : 512;
+  return L >= 4 ? 256 : 512;
 }
 
 // Rounds with at most this many pairs run as a fold launch plus the point-parallel evaluation
