@@ -340,8 +340,9 @@ struct Work<FieldP256> {
     for (int i = 0; i < 9; ++i) s.l[i] = lo.l[i] + m.l[i];
     return r29_reduce(s);
   }
-  // Running sum of products (each < 16p): limbs 0..7 normalized, the top limb reduced
-  // before it can reach 2^32.
+  // Running sum of terms: products (< 13p) and single factors, which are interpolated
+  // values x(t) < 29p + 2^244 (t <= 7). Limbs 0..7 normalized, the top limb reduced before it
+  // can reach 2^32.
   struct Acc {
     R29 v;
     SC_HD void clear() { v = r29_zero(); }
@@ -351,9 +352,11 @@ struct Work<FieldP256> {
       v = r29_normalize(v);
       if (v.l[8] >= (1u << 31)) v = r29_reduce(v);
     }
-    SC_HD void sub(const R29& x) {  // adds 17p - x
+    // Adds 33p - x for any normalized x below 32p: with a fresh (zero) accumulator the top
+    // limb of 33p - x must not borrow, and x(7) of a single-factor term comes to 29p.
+    SC_HD void sub(const R29& x) {
 #pragma unroll
-      for (int i = 0; i < 9; ++i) v.l[i] += r29::PAD(17, i) - x.l[i];
+      for (int i = 0; i < 9; ++i) v.l[i] += r29::PAD(33, i) - x.l[i];
       v = r29_normalize(v);
       if (v.l[8] >= (1u << 31)) v = r29_reduce(v);
     }

@@ -42,6 +42,8 @@ extern "C" void field_host_op(int bit_width, int op, uint64_t n, const uint32_t*
 //   op 3: sum of 40 products a*b minus one, through the accumulator (39 a*b)
 //   op 4: a*b - c*x(t) for every t in 0..7, summed (x(t) = a + t (b - a))
 //   op 5: a x^2 + c a, x = a + 7 (b - a), the last two products sharing one reduction
+//   op 6: -x(t) into a fresh accumulator, t = i mod 8, x(t) reached by interpolation steps
+//   op 7: -x(t) into a fresh accumulator, t = i mod 8, x(t) = at(a, b - a, t)
 template <class F>
 static void run_work(int op, uint64_t n, const uint32_t* a, const uint32_t* b, const uint32_t* c, uint32_t* out) {
   using W = Work<F>;
@@ -70,6 +72,18 @@ static void run_work(int op, uint64_t n, const uint32_t* a, const uint32_t* b, c
       typename W::V t = y;
       for (int k = 2; k <= 7; ++k) t = W::step(t, d);
       acc.add(W::mul_sum(W::mul(x, t), t, z, x));  // a x7^2 + c a with x7 = a + 7 (b - a)
+    } else if (op == 6 || op == 7) {
+      // a single-factor term with coefficient -1 is the first term a thread adds: the
+      // subtraction must hold for the largest interpolated value on its own
+      const int tt = (int)(i % 8);
+      const typename W::V d = W::diff(y, x);
+      typename W::V t = x;
+      if (op == 7) {
+        t = W::at(x, d, tt);
+      } else {
+        for (int k = 1; k <= tt; ++k) t = k == 1 ? y : W::step(t, d);
+      }
+      acc.sub(t);
     } else {
       const typename W::V d = W::diff(y, x);
       typename W::V t = x;

@@ -140,6 +140,11 @@ def test_host_working_form(harness, bw):
         want4.append((8 * mont(x, y) - mont(z, xs % p)) % p)
     assert harness.work(bw, 4, a, b, c) == want4
     assert harness.work(bw, 5, a, b, c) == [(mont(mont(x, t), t) + mont(z, x)) % p for x, t, z in zip(a, x7, c)]
+    # -x(t) alone in a fresh accumulator (a term like `- g` of a degree-7 composition): x(7)
+    # of the 255-bit working form comes to 29p, so the subtraction's padding must exceed it
+    want6 = [(-(x + (i % 8) * (y - x))) % p for i, (x, y) in enumerate(zip(a, b))]
+    assert harness.work(bw, 6, a, b, c) == want6
+    assert harness.work(bw, 7, a, b, c) == want6
 
 
 def test_host_goldilocks_product_sum_one_reduction(harness):
