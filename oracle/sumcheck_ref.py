@@ -67,19 +67,32 @@ def unmont(x: int, p: int, limbs: int) -> int:
 # Expression rules: src/zkduel/_expr.py:27-67
 # ---------------------------------------------------------------------------------------------
 
+def _coefficient_term(term):
+    """(int(c), names) when term is a two-item list or tuple whose first item is not a string
+    and converts with int() (ints, bools, numpy integers, floats truncated) and whose second
+    item is a non-string iterable of strings (_expr.py:11-24); else None."""
+    if not isinstance(term, (list, tuple)) or len(term) != 2:
+        return None
+    c, names = term
+    if isinstance(c, str) or isinstance(names, str):
+        return None
+    try:
+        coeff = int(c)
+        names = tuple(names)
+    except (TypeError, ValueError):
+        return None
+    if not all(isinstance(v, str) for v in names):
+        return None
+    return coeff, names
+
+
 def normalize_expression(expression) -> tuple[tuple[int, tuple[str, ...]], ...]:
     """Terms are ``[names]`` (coefficient 1) or ``(coeff, [names])`` (_expr.py:27-54)."""
     out = []
     for term in expression:
-        if (
-            isinstance(term, (list, tuple))
-            and len(term) == 2
-            and not isinstance(term[0], str)
-            and not isinstance(term[1], str)
-            and isinstance(term[0], int)
-            and all(isinstance(v, str) for v in term[1])
-        ):
-            out.append((int(term[0]), tuple(term[1])))
+        weighted = _coefficient_term(term)
+        if weighted is not None:
+            out.append(weighted)
             continue
         if isinstance(term, str):
             raise TypeError("Expression terms must be sequences of variable names")

@@ -25,6 +25,24 @@ def montgomery_nprime(modulus: int) -> int:
     return (-pow(int(modulus), -1, 1 << 32)) & MASK32
 
 
+def _unsupported_reason(modulus: int, limbs: int, nprime: int) -> str | None:
+    """Why the library cannot run Montgomery arithmetic with these parameters, or None.
+
+    The device kernels need an odd modulus of at most 32 * limbs bits and the nprime the
+    Montgomery product uses; any such modulus is exact (they keep the full carry word)."""
+    bits = 32 * limbs
+    if limbs <= 0:
+        return "the limb count must be positive"
+    if modulus <= 2 or not modulus & 1:
+        return "Montgomery form needs an odd modulus above 2"
+    if modulus.bit_length() > bits:
+        return f"the modulus has {modulus.bit_length()} bits, more than {bits}"
+    want = montgomery_nprime(modulus)
+    if nprime != want:
+        return f"nprime is {nprime:#x}; -p^-1 mod 2^32 is {want:#x}"
+    return None
+
+
 @dataclass(frozen=True)
 class FieldSpec:
     name: str
@@ -34,16 +52,9 @@ class FieldSpec:
     nprime: int
 
     def __post_init__(self) -> None:
-        p, L = int(self.modulus), int(self.limbs)
-        where = f"modulus {p:#x} with {L} x 32-bit limbs"
-        if L < 1:
-            raise UnsupportedFieldError(f"{where}: need at least one limb")
-        if p < 3 or p % 2 == 0:
-            raise UnsupportedFieldError(f"{where}: Montgomery arithmetic needs an odd modulus > 2")
-        if p >= 1 << (32 * L):
-            raise UnsupportedFieldError(f"{where}: modulus does not fit in {L} limbs")
-        if int(self.nprime) != montgomery_nprime(p):
-            raise UnsupportedFieldError(f"{where}: nprime {self.nprime:#x} != -p^-1 mod 2^32")
+        reason = _unsupported_reason(int(self.modulus), int(self.limbs), int(self.nprime))
+        if reason is not None:
+            raise UnsupportedFieldError(f"field {self.name!r} (p = {int(self.modulus):#x}): {reason}")
 
     @property
     def r_mod(self) -> int:
@@ -98,13 +109,12 @@ FIELD_SPECS: dict[int, FieldSpec] = {
 
 
 def get_field(bit_width: int) -> FieldSpec:
-    try:
-        return FIELD_SPECS[int(bit_width)]
-    except KeyError as exc:
-        supported = ", ".join(str(x) for x in sorted(FIELD_SPECS))
-        raise ValueError(
-            f"unsupported field bit width {bit_width}; expected one of {supported}"
-        ) from exc
+    """The field of the given width (32, 64, 128 or 256); ValueError for any other."""
+    spec = FIELD_SPECS.get(int(bit_width))
+    if spec is None:
+        widths = "/".join(map(str, FIELD_SPECS))
+        raise ValueError(f"no field of bit width {bit_width}: the library has {widths}")
+    return spec
 
 
 def field_for_modulus(q: int) -> FieldSpec:

@@ -86,6 +86,44 @@ def test_expression_rules_follow_reference():
         normalize_expression(["ab"])
 
 
+def test_expression_rule_edge_cases():
+    """Each rule of the reference's _expr.py:11-54, case by case: what counts as a weighted
+    term, what is a plain term of coefficient 1, and what is an error."""
+    import numpy as np
+
+    # int()-convertible coefficients: numpy integers, bools, floats (truncated)
+    expr = [(np.int64(5), ["a", "b"]), (np.uint32(2), ("c",)), (True, ["a"]), (2.9, ["b"])]
+    assert normalize_expression(expr) == ((5, ("a", "b")), (2, ("c",)), (1, ("a",)), (2, ("b",)))
+    # a two-item term of two names is a plain product, not a weighted term
+    assert normalize_expression([["x", "y"], ("y", "z")]) == ((1, ("x", "y")), (1, ("y", "z")))
+    # a string coefficient or a string of names is not a weighted term: the pair is then a
+    # plain term holding a non-name, so an error
+    with pytest.raises(TypeError):
+        normalize_expression([("3", ["a"])])
+    with pytest.raises(TypeError):
+        normalize_expression([(3, "ab")])
+    with pytest.raises(TypeError):
+        normalize_expression([(None, ["a"])])
+    with pytest.raises(TypeError):
+        normalize_expression([5])
+    with pytest.raises(TypeError):
+        normalize_expression([["a", 1, "b"]])
+    # repeated variables are powers; order is first appearance across terms
+    names, coeffs, lens, offs, flat = encode_expression_with_coeffs([["b", "b", "a"], (-2, ["c", "b"])])
+    assert names == ["b", "a", "c"] and coeffs == (1, -2) and lens == (3, 2) and offs == (0, 3)
+    assert flat == (0, 0, 1, 2, 0)
+    assert expression_degree([["a"], (4, ["a", "a", "a", "b"])]) == 4
+    # the oracle's restatement follows the same rules
+    from oracle import sumcheck_ref as R
+
+    for e in (expr, [["x", "y"], ("y", "z")], [["b", "b", "a"], (-2, ["c", "b"])]):
+        assert R.normalize_expression(e) == normalize_expression(e)
+        assert R.expression_vars(e) == expression_vars(e)
+    for bad in ([("3", ["a"])], [(3, "ab")], [5], [["a", 1]]):
+        with pytest.raises(TypeError):
+            R.normalize_expression(bad)
+
+
 def test_field_specs():
     for bw, f in FIELD_SPECS.items():
         assert f.limbs * 32 == bw
