@@ -31,6 +31,11 @@ def load():
             ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, u32p, ctypes.c_int,
             u32p, ctypes.c_int, u32p, u32p, u32p,
         ]
+        lib.oracle_claim_final.restype = ctypes.c_int
+        lib.oracle_claim_final.argtypes = [
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, u32p, ctypes.c_int,
+            ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, u32p, ctypes.c_int, u32p, u32p,
+        ]
         lib.oracle_sha3_256.restype = None
         lib.oracle_sha3_256.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_char_p]
         lib.oracle_mont_mul.restype = None
@@ -75,6 +80,30 @@ def prove(bit_width, num_tables, terms, num_vars, *, input_kind="uniform", seed=
     if rc != 0:
         raise RuntimeError(f"oracle_prove failed ({rc})")
     return polys, rr[: num_vars - 1], final
+
+
+def claim_and_final(bit_width, num_tables, terms, num_vars, point_mont, *, seed=1, instance=0,
+                    selector_mask=0, threads=None):
+    """For the uniform tables keyed by (seed, instance): (claim [L], final [T][L]) in Montgomery
+    limbs: the sum of the composition over the hypercube, and every table's multilinear
+    extension at point_mont ([num_vars][L] Montgomery, variable 0 = the first round's)."""
+    lib = load()
+    p, L = R.field(bit_width)
+    lens = np.array([len(idx) for _c, idx in terms], dtype=np.int32)
+    offs = np.array(np.cumsum([0] + [len(idx) for _c, idx in terms])[:-1], dtype=np.int32)
+    flat = np.array([i for _c, idx in terms for i in idx], dtype=np.int32)
+    coeffs = _limbs([c % p for c, _idx in terms], L)
+    point = np.ascontiguousarray(point_mont, dtype=np.uint32).reshape(num_vars, L)
+    claim = np.zeros(L, dtype=np.uint32)
+    final = np.zeros((num_tables, L), dtype=np.uint32)
+    threads = threads or min(16, os.cpu_count() or 1)
+    rc = lib.oracle_claim_final(
+        bit_width, num_tables, len(terms), lens.ctypes.data_as(i32p), offs.ctypes.data_as(i32p),
+        flat.ctypes.data_as(i32p), coeffs.ctypes.data_as(u32p), num_vars, seed, instance, selector_mask,
+        point.ctypes.data_as(u32p), threads, claim.ctypes.data_as(u32p), final.ctypes.data_as(u32p))
+    if rc != 0:
+        raise RuntimeError(f"oracle_claim_final failed ({rc})")
+    return claim, final
 
 
 def sha3_256(data: bytes) -> bytes:

@@ -449,3 +449,134 @@ void oracle_mont_mul(int bit_width, uint64_t n, const uint32_t* a, const uint32_
   field_init(&f, bit_width);
   for (uint64_t i = 0; i < n; ++i) fmul(&f, out + i * f.L, a + i * f.L, b + i * f.L);
 }
+
+/* ---------------------------------------------------------------- claim and final point */
+
+typedef struct {
+  const field_t* f;
+  int T, K;
+  const int* exps;
+  const uint32_t* coeff;
+  uint64_t seed, selector_mask;
+  uint32_t instance;
+  uint64_t lo, hi; /* pairs */
+  const uint32_t* r0;
+  uint32_t* folded; /* [T][n0/2][L] */
+  uint64_t half;
+  uint32_t claim[MAXL];
+} claim_job_t;
+
+static void gen_mont(const field_t* f, uint64_t seed, uint32_t inst, uint64_t sel_mask, int v, uint64_t i,
+                     uint32_t* out) {
+  uint32_t x[MAXL] = {0};
+  const uint64_t key = key_of(seed, inst, (uint32_t)v);
+  if ((sel_mask >> v) & 1) x[0] = (uint32_t)(draw(key, i * 128) & 1);
+  else uniform_canon(f, key, i, x);
+  to_mont(f, out, x);
+}
+
+static void* run_claim_job(void* arg) {
+  claim_job_t* j = (claim_job_t*)arg;
+  const field_t* f = j->f;
+  const int L = f->L;
+  uint32_t a[MAXT * MAXL], b[MAXT * MAXL], c[MAXL], tmp[MAXL];
+  job_t shape; /* compose() reads only f, T, K, exps and coeff */
+  memset(&shape, 0, sizeof(shape));
+  shape.f = f; shape.T = j->T; shape.K = j->K; shape.exps = j->exps; shape.coeff = j->coeff;
+  memset(j->claim, 0, sizeof(j->claim));
+  for (uint64_t m = j->lo; m < j->hi; ++m) {
+    for (int v = 0; v < j->T; ++v) {
+      gen_mont(f, j->seed, j->instance, j->selector_mask, v, 2 * m, a + v * L);
+      gen_mont(f, j->seed, j->instance, j->selector_mask, v, 2 * m + 1, b + v * L);
+      fsub(f, tmp, b + v * L, a + v * L);
+      fmul(f, tmp, tmp, j->r0);
+      fadd(f, j->folded + ((size_t)v * j->half + m) * L, a + v * L, tmp);
+    }
+    compose(&shape, a, c);
+    fadd(f, j->claim, j->claim, c);
+    compose(&shape, b, c);
+    fadd(f, j->claim, j->claim, c);
+  }
+  return NULL;
+}
+
+typedef struct {
+  const field_t* f;
+  uint32_t* t; /* one table, n entries */
+  uint64_t n;
+  const uint32_t* point; /* [num_vars][L], point[0] already applied */
+  int num_vars;
+} mle_job_t;
+
+static void* run_mle_job(void* arg) {
+  mle_job_t* j = (mle_job_t*)arg;
+  const field_t* f = j->f;
+  const int L = f->L;
+  uint32_t tmp[MAXL];
+  uint64_t n = j->n;
+  for (int k = 1; k < j->num_vars; ++k, n /= 2) {
+    for (uint64_t m = 0; m < n / 2; ++m) {
+      const uint32_t* a = j->t + 2 * m * L;
+      const uint32_t* b = a + L;
+      fsub(f, tmp, b, a);
+      fmul(f, tmp, tmp, j->point + (size_t)k * L);
+      fadd(f, j->t + m * L, a, tmp);
+    }
+  }
+  return NULL;
+}
+
+/* For the uniform (counter-based) tables keyed by (seed, instance), selector tables in {0, 1}:
+ * out_claim [L] = sum over the hypercube of the composition, and out_final [T][L] = every
+ * table's multilinear extension at point_mont [num_vars][L] (variable 0 = the lowest index
+ * bit, the first round's challenge), all Montgomery. Tables are generated on the fly and
+ * folded once by the first coordinate, so memory is half the input. Returns 0 on success. */
+int oracle_claim_final(int bit_width, int T, int K, const int32_t* term_lens, const int32_t* term_offsets,
+                       const int32_t* term_vars, const uint32_t* coeffs_canon, int num_vars, uint64_t seed,
+                       uint32_t instance, uint64_t selector_mask, const uint32_t* point_mont, int threads,
+                       uint32_t* out_claim, uint32_t* out_final) {
+  if (bit_width != 32 && bit_width != 64 && bit_width != 128 && bit_width != 256) return 1;
+  if (T < 1 || T > MAXT || K < 1 || num_vars < 1 || num_vars > 40) return 1;
+  field_t f;
+  field_init(&f, bit_width);
+  const int L = f.L;
+  int* exps = calloc((size_t)K * T, sizeof(int));
+  uint32_t* coeff = calloc((size_t)K * L, 4);
+  for (int k = 0; k < K; ++k) {
+    for (int m = 0; m < term_lens[k]; ++m) exps[k * T + term_vars[term_offsets[k] + m]]++;
+    to_mont(&f, coeff + (size_t)k * L, coeffs_canon + (size_t)k * L);
+  }
+  const uint64_t half = (uint64_t)1 << (num_vars - 1);
+  uint32_t* folded = malloc((size_t)T * half * L * 4);
+  if (!folded) { free(exps); free(coeff); return 2; }
+  if (threads < 1) threads = 1;
+  int nt = (uint64_t)threads > half ? (int)half : threads;
+  claim_job_t* jobs = calloc((size_t)nt, sizeof(claim_job_t));
+  pthread_t* tid = calloc((size_t)(nt > T ? nt : T), sizeof(pthread_t));
+  for (int t = 0; t < nt; ++t) {
+    claim_job_t* jb = &jobs[t];
+    jb->f = &f; jb->T = T; jb->K = K; jb->exps = exps; jb->coeff = coeff;
+    jb->seed = seed; jb->selector_mask = selector_mask; jb->instance = instance;
+    jb->lo = half * t / nt; jb->hi = half * (t + 1) / nt; jb->r0 = point_mont;
+    jb->folded = folded; jb->half = half;
+    pthread_create(&tid[t], NULL, run_claim_job, jb);
+  }
+  uint32_t claim[MAXL] = {0};
+  for (int t = 0; t < nt; ++t) {
+    pthread_join(tid[t], NULL);
+    fadd(&f, claim, claim, jobs[t].claim);
+  }
+  memcpy(out_claim, claim, 4 * L);
+  mle_job_t* mj = calloc((size_t)T, sizeof(mle_job_t));
+  for (int v = 0; v < T; ++v) {
+    mj[v].f = &f; mj[v].t = folded + (size_t)v * half * L; mj[v].n = half;
+    mj[v].point = point_mont; mj[v].num_vars = num_vars;
+    pthread_create(&tid[v], NULL, run_mle_job, &mj[v]);
+  }
+  for (int v = 0; v < T; ++v) {
+    pthread_join(tid[v], NULL);
+    memcpy(out_final + (size_t)v * L, folded + (size_t)v * half * L, 4 * L);
+  }
+  free(mj); free(jobs); free(tid); free(folded); free(exps); free(coeff);
+  return 0;
+}

@@ -78,3 +78,31 @@ def test_c_twin_reproduces_sweep_checksum_at_scale(bw):
     T, terms = terms_of("poly_abc_plus_de")
     polys, _rr, _f = C.prove(bw, T, terms, 18, input_kind="affine", seed=19, challenge_mode="fixed")
     assert checksum_of(polys) * g["runs"] == g["checksum_sum"]
+
+
+def _mle(values, point, p):
+    """Multilinear extension by its definition: sum_i f_i prod_k (r_k if bit k of i else 1 - r_k)."""
+    total = 0
+    for i, x in enumerate(values):
+        w = 1
+        for k, r in enumerate(point):
+            w = w * (r if (i >> k) & 1 else 1 - r) % p
+        total += x * w
+    return total % p
+
+
+@pytest.mark.parametrize("bw", [32, 64, 128, 256])
+def test_c_claim_and_final_point(bw):
+    """oracle_claim_final (the full-size checks' claim and final point) against the Python
+    oracle's claim and the multilinear extension by its definition, with a selector table."""
+    p, L = R.field(bw)
+    T, terms = terms_of("zk_complete_add_1")
+    rng = random.Random(bw + 5)
+    for nv in (1, 4, 7):
+        point = [rng.randrange(p) for _ in range(nv)]
+        tables = R.uniform_tables(T, nv, 11, p, L, selectors=(1,))
+        claim, _g, _r = R.prove(tables, terms, p, nv, R.sha3_challenger(p, L))
+        pm = [R.to_limbs(R.mont(x, p, L), L) for x in point]
+        c, fin = C.claim_and_final(bw, T, terms, nv, pm, seed=11, selector_mask=2, threads=3)
+        assert R.unmont(R.from_limbs(c), p, L) == claim, nv
+        assert [R.unmont(R.from_limbs(x), p, L) for x in fin] == [_mle(t, point, p) for t in tables], nv
