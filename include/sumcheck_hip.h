@@ -154,6 +154,26 @@ int sc_prove(sc_context* ctx, int32_t bit_width, const sc_composition* comp, int
              const sc_input* input, int32_t challenge_mode, const uint32_t* explicit_r,
              uint32_t* round_polys, uint32_t* challenges, uint64_t* checksum);
 
+/* sc_prove without allocation: no counterpart in the reference. For callers that cannot
+ * allocate device memory: code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc. It allocates nothing (no hipMalloc/hipFree, no host copy) and enqueues the
+ * whole proof on the context's stream without waiting (sc_context_sync waits). All buffers are
+ * device memory of the caller:
+ *   tables       [T][2^num_vars][L] Montgomery, element-major, read only (SC_INPUT_DEVICE_MONT)
+ *   round_polys  [num_vars][degree+1][L] Montgomery, out
+ *   challenges   [num_vars-1][L] Montgomery, out, may be NULL
+ *   checksum     one uint64, out, may be NULL (as sc_prove)
+ * challenge_mode SC_CHALLENGE_FIXED or SC_CHALLENGE_SHA3 (else SC_ERR_UNSUPPORTED). The
+ * intermediate tables, block partials, descriptor and round state live in one fixed device
+ * array in the library, one per GPU, held until the process ends: the intermediate tables
+ * (0.75x the input) must fit in SC_NOALLOC_FOLD_BYTES, else SC_ERR_UNSUPPORTED. Every call on
+ * a GPU uses that GPU's array: calls that overlap on one GPU (other contexts or streams) race;
+ * calls on one context are ordered by its stream. */
+#define SC_NOALLOC_FOLD_BYTES (128u << 20)
+int sc_prove_noalloc(sc_context* ctx, int32_t bit_width, const sc_composition* comp, int32_t num_vars,
+                     const uint32_t* tables, int32_t challenge_mode, uint32_t* round_polys,
+                     uint32_t* challenges, uint64_t* checksum);
+
 /* Elementwise device field operations on n elements (host buffers, [n][L]):
  * op 0: a*b*R^-1 (Montgomery product), 1: a+b, 2: a-b, 3: a -> a*R (encode),
  * 4: a -> a*R^-1 (decode). The device twin of src/zkduel/fields.py:63-115, and what the

@@ -13,6 +13,7 @@
 #include <deque>
 #include <map>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -523,6 +524,25 @@ int upload_explicit(sc_proof* p, int mode, const uint32_t* explicit_r) {
   return SC_OK;
 }
 
+// Shape of a proof and its default schedule (library defaults, as measured above).
+void init_shape(sc_proof* p, sc_context* ctx, int fidx, const CompDesc& desc, int num_vars) {
+  p->ctx = ctx;
+  p->fidx = fidx;
+  p->ln = &launchers_for(fidx);
+  p->L = p->ln->limbs;
+  p->T = desc.num_tables;
+  p->npts = desc.npts;
+  p->num_vars = num_vars;
+  p->tmax = pick_tmax(p->T);
+  p->pmax = p->npts <= 4 ? 4 : 8;
+  p->n0 = (uint64_t)1 << num_vars;
+  p->tail_pairs = default_proof_tail_pairs(p->L, p->tmax > 8);
+  p->max_pass_blocks = default_pass_blocks(p->L);
+  p->matched_sid = match_static(desc);
+  p->sid = p->matched_sid;
+  p->wide_pairs = default_wide_pairs(p);
+}
+
 void free_proof(sc_proof* p) {
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
@@ -538,6 +558,83 @@ void free_proof(sc_proof* p) {
   for (hipEvent_t e : p->prof_ev)
     if (e) (void)hipEventDestroy(e);
   delete p;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Device memory of sc_prove_noalloc, the entry point for callers that cannot allocate: round
+// 1's and round 2's tables (0.75x the input at most), then the block partials, the
+// composition descriptor and the round state, in one fixed array per GPU.
+// ---------------------------------------------------------------------------------------------
+constexpr uint64_t NOALLOC_FOLD_WORDS = (uint64_t)SC_NOALLOC_FOLD_BYTES / 4;
+constexpr uint64_t NOALLOC_PARTIAL_WORDS = (uint64_t)MAX_PASS_BLOCKS * MAX_POINTS * MAX_LIMBS;
+constexpr uint64_t round_up256(uint64_t b) { return (b + 255) & ~uint64_t(255); }
+constexpr uint64_t NOALLOC_OFF_PARTIALS = NOALLOC_FOLD_WORDS * 4;
+constexpr uint64_t NOALLOC_OFF_COMP = NOALLOC_OFF_PARTIALS + NOALLOC_PARTIAL_WORDS * 4;
+constexpr uint64_t NOALLOC_OFF_STATE = NOALLOC_OFF_COMP + round_up256(sizeof(CompDesc));
+constexpr uint64_t NOALLOC_BYTES = NOALLOC_OFF_STATE + round_up256(sizeof(ProofState));
+
+}  // namespace
+
+// In host code `scratch` is a host-side stand-in, not device memory: host code uses only
+// the per-device address that hipGetSymbolAddress returns (g_scratch), never this name.
+static __device__ __align__(256) uint8_t scratch[NOALLOC_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch[MAX_DEVICES] = {};
+std::string g_scratch_error[MAX_DEVICES];
+
+// Called by sc_context_create with `device` current: resolves this GPU's copy of scratch once.
+// A failure does not fail the context; sc_prove_noalloc then reports it.
+void resolve_scratch(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device] != nullptr) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr) {
+    g_scratch_error[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return;
+  }
+  g_scratch[device] = static_cast<uint8_t*>(ptr);
+}
+
+// Writes the composition descriptor (passed by value as the kernel argument, so nothing is
+// copied from host memory) and clears the round state with vector stores.
+__global__ void __launch_bounds__(BLOCK) noalloc_setup_kernel(CompDesc desc, CompDesc* comp, ProofState* st) {
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(&desc);
+  uint32_t* dc = reinterpret_cast<uint32_t*>(comp);
+  for (uint32_t k = threadIdx.x; k < sizeof(CompDesc) / 4; k += BLOCK) dc[k] = src[k];
+  uint32_t* ds = reinterpret_cast<uint32_t*>(st);
+  for (uint32_t k = threadIdx.x; k < sizeof(ProofState) / 4; k += BLOCK) ds[k] = 0u;
+}
+
+// The proof's outputs from its round state: round_polys [num_vars][npts][L], challenges
+// [num_vars-1][L] (may be NULL) and checksum sum_j sum_t sum_i (i + 1) limb_i (may be NULL),
+// as sc_proof_result forms them (cuda/src/field_sumcheck.cu:1043-1049).
+__global__ void __launch_bounds__(BLOCK) noalloc_export_kernel(const ProofState* st, int num_vars, int npts, int L,
+                                                              uint32_t* polys, uint32_t* chal, uint64_t* checksum) {
+  __shared__ uint64_t s_sum[BLOCK];
+  const int per_round = npts * L;
+  uint64_t sum = 0;
+  for (int k = threadIdx.x; k < num_vars * per_round; k += BLOCK) {
+    const int j = k / per_round, t = (k % per_round) / L, i = k % L;
+    const uint32_t w = st->polys[j][t][i];
+    polys[k] = w;
+    sum += (uint64_t)(i + 1) * w;
+  }
+  if (chal != nullptr) {
+    for (int k = threadIdx.x; k < (num_vars - 1) * L; k += BLOCK) chal[k] = st->r[k / L][k % L];
+  }
+  s_sum[threadIdx.x] = sum;
+  __syncthreads();
+  for (int half = BLOCK / 2; half > 0; half >>= 1) {
+    if (threadIdx.x < half) s_sum[threadIdx.x] += s_sum[threadIdx.x + half];
+    __syncthreads();
+  }
+  if (checksum != nullptr && threadIdx.x == 0) *checksum = s_sum[0];
 }
 
 }  // namespace
@@ -558,6 +655,7 @@ int sc_context_create(int device, sc_context** out) {
   sc_context* c = new (std::nothrow) sc_context();
   if (c == nullptr) return fail(SC_ERR_MEMORY, "host allocation failed");
   c->device = device;
+  resolve_scratch(device);
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;
@@ -613,21 +711,7 @@ int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* co
   SC_HIP(hipSetDevice(ctx->device));
   sc_proof* p = new (std::nothrow) sc_proof();
   if (p == nullptr) return fail(SC_ERR_MEMORY, "host allocation failed");
-  p->ctx = ctx;
-  p->fidx = fidx;
-  p->ln = &launchers_for(fidx);
-  p->L = p->ln->limbs;
-  p->T = desc.num_tables;
-  p->npts = desc.npts;
-  p->num_vars = num_vars;
-  p->tmax = pick_tmax(p->T);
-  p->pmax = p->npts <= 4 ? 4 : 8;
-  p->n0 = (uint64_t)1 << num_vars;
-  p->tail_pairs = default_proof_tail_pairs(p->L, p->tmax > 8);
-  p->max_pass_blocks = default_pass_blocks(p->L);
-  p->matched_sid = match_static(desc);
-  p->sid = p->matched_sid;
-  p->wide_pairs = default_wide_pairs(p);
+  init_shape(p, ctx, fidx, desc, num_vars);
 
   const uint64_t words = (uint64_t)p->T * p->n0 * p->L;
   auto cleanup = [&](int code) {
@@ -907,6 +991,51 @@ int sc_prove(sc_context* ctx, int32_t bit_width, const sc_composition* comp, int
   sc_proof_destroy(p);
   g_error = err;
   return rc;
+}
+
+int sc_prove_noalloc(sc_context* ctx, int32_t bit_width, const sc_composition* comp, int32_t num_vars,
+                     const uint32_t* tables, int32_t mode, uint32_t* round_polys, uint32_t* challenges,
+                     uint64_t* checksum) {
+  if (ctx == nullptr || tables == nullptr || round_polys == nullptr)
+    return fail(SC_ERR_ARG, "ctx, tables and round_polys must not be NULL");
+  const int fidx = field_index(bit_width);
+  if (fidx < 0) return fail(SC_ERR_ARG, "bit_width must be 32, 64, 128 or 256");
+  if (num_vars < 1 || num_vars > MAX_ROUNDS) return fail(SC_ERR_UNSUPPORTED, "num_vars must be in [1, 40]");
+  if (mode != SC_CHALLENGE_FIXED && mode != SC_CHALLENGE_SHA3)
+    return fail(SC_ERR_UNSUPPORTED, "sc_prove_noalloc takes challenge_mode 0 (fixed) or 1 (sha3)");
+  CompDesc desc;
+  int degree = 0;
+  int rc = build_comp_any(fidx, comp, ctx->host_fc[fidx], desc, degree);
+  if (rc != SC_OK) return rc;
+  const uint64_t L = (uint64_t)launchers_for(fidx).limbs;
+  const uint64_t n0 = (uint64_t)1 << num_vars;
+  const uint64_t fold_words = (uint64_t)desc.num_tables * (n0 / 2 + n0 / 4) * L;
+  if (fold_words > NOALLOC_FOLD_WORDS)
+    return fail(SC_ERR_UNSUPPORTED, "sc_prove_noalloc: the intermediate tables (0.75x the input, " +
+                                        std::to_string(fold_words * 4) + " bytes) exceed the library's fixed " +
+                                        std::to_string(NOALLOC_FOLD_WORDS * 4) + "-byte area");
+  if (ctx->device < 0 || ctx->device >= MAX_DEVICES) return fail(SC_ERR_UNSUPPORTED, "device index above 63");
+  uint8_t* base = g_scratch[ctx->device];
+  if (base == nullptr) return fail(SC_ERR_HIP, g_scratch_error[ctx->device]);
+  SC_HIP(hipSetDevice(ctx->device));
+  // A host-side description only: every device pointer is the caller's memory or scratch.
+  sc_proof p;
+  init_shape(&p, ctx, fidx, desc, num_vars);
+  p.d_tables = const_cast<uint32_t*>(tables);
+  p.d_a = reinterpret_cast<uint32_t*>(base);
+  p.d_b = p.d_a + (uint64_t)p.T * (n0 / 2) * L;
+  p.d_partials = reinterpret_cast<uint32_t*>(base + NOALLOC_OFF_PARTIALS);
+  p.d_comp = reinterpret_cast<CompDesc*>(base + NOALLOC_OFF_COMP);
+  p.d_state = reinterpret_cast<ProofState*>(base + NOALLOC_OFF_STATE);
+  hipStream_t s = ctx->stream;
+  hipLaunchKernelGGL(noalloc_setup_kernel, dim3(1), dim3(BLOCK), 0, s, desc, p.d_comp, p.d_state);
+  SC_HIP(hipGetLastError());
+  rc = enqueue_rounds(&p, mode);
+  if (rc != SC_OK) return rc;
+  hipLaunchKernelGGL(noalloc_export_kernel, dim3(1), dim3(BLOCK), 0, s, (const ProofState*)p.d_state, num_vars,
+                     p.npts, (int)L, round_polys, challenges, checksum);
+  SC_HIP(hipGetLastError());
+  return SC_OK;
 }
 
 int sc_field_op(sc_context* ctx, int32_t bit_width, int32_t op, uint64_t n, const uint32_t* a,

@@ -87,6 +87,9 @@ SIGNATURES = [
     ("sc_prove", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ScComposition), ctypes.c_int32,
       ctypes.POINTER(ScInput), ctypes.c_int32, u32p, u32p, u32p, u64p]),
+    ("sc_prove_noalloc", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ScComposition), ctypes.c_int32,
+      ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("sc_field_op", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, u32p, u32p, u32p]),
     ("sc_batch_create", ctypes.c_int,
