@@ -50,6 +50,36 @@ constexpr int static_degree(int id) {
     if (STATIC_COMPS[id].lens[k] > d) d = STATIC_COMPS[id].lens[k];
   return d;
 }
+
+// Round 0 of a compiled composition in the 255-bit field reads the caller's Montgomery words
+// X = x 2^256 directly as working values, which stand for x 2^-5 (the working form holds
+// x 2^261), saving each input its reduction. A product of len such factors then carries
+// 32^-len. Each term's first (len - S) factors are read exactly instead, S the shortest
+// term's length, so every term carries 32^-S; the round's sum is multiplied by 32^S once,
+// when its block partials are reduced. (The compiled compositions' terms share no tables.)
+constexpr int static_round0_scale(int id) {
+  int s = 8;
+  for (int k = 0; k < STATIC_COMPS[id].num_terms; ++k)
+    if (STATIC_COMPS[id].lens[k] < s) s = STATIC_COMPS[id].lens[k];
+  return s;
+}
+constexpr bool static_round0_exact(int id, int v) {
+  const int s = static_round0_scale(id);
+  for (int k = 0; k < STATIC_COMPS[id].num_terms; ++k)
+    for (int m = 0; m < STATIC_COMPS[id].lens[k] - s; ++m)
+      if (STATIC_COMPS[id].vars[k][m] == v) return true;
+  return false;
+}
+constexpr bool static_terms_disjoint(int id) {
+  for (int k = 0; k < STATIC_COMPS[id].num_terms; ++k)
+    for (int m = 0; m < STATIC_COMPS[id].lens[k]; ++m)
+      for (int k2 = k + 1; k2 < STATIC_COMPS[id].num_terms; ++k2)
+        for (int m2 = 0; m2 < STATIC_COMPS[id].lens[k2]; ++m2)
+          if (STATIC_COMPS[id].vars[k][m] == STATIC_COMPS[id].vars[k2][m2]) return false;
+  return true;
+}
+static_assert(static_terms_disjoint(1) && static_terms_disjoint(2) && static_terms_disjoint(3),
+              "the round-0 scaling reads each table exactly or scaled for all its uses");
 enum ChallengeMode : int32_t { CHALLENGE_FIXED = 0, CHALLENGE_SHA3 = 1, CHALLENGE_EXPLICIT = 2 };
 enum FillKind : int32_t { FILL_AFFINE = 0, FILL_UNIFORM = 1 };
 
@@ -72,6 +102,7 @@ struct FieldConsts {
   // Barycentric weights: bary[n-1][t] = 1 / prod_{s != t, s < n} (t - s), Montgomery, so
   // g(r) = sum_t g(t) bary[n-1][t] prod_{s != t} (r - s) for a polynomial known at 0..n-1.
   uint32_t bary[MAX_POINTS][MAX_POINTS][MAX_LIMBS];
+  uint32_t pow32[MAX_POINTS][MAX_LIMBS];  // Montgomery images of 32^k, k < 8
 };
 
 // Per-proof device state. polys and r are written by the kernel that finishes a round.
@@ -128,6 +159,8 @@ struct FinalizeArgs {
   int32_t round;
   int32_t num_rounds;
   int32_t challenge_mode;
+  int32_t scale_pow;  // the partials sum to g / 32^scale_pow (static_round0_scale), else 0
+  int32_t pad0;
   const CompDesc* comp;
   const FieldConsts* fc;
   ProofState* st;

@@ -111,6 +111,11 @@ struct HostField {
         for (int i = 0; i < L; ++i) fc.bary[n - 1][t][i] = w.v[i];
       }
     }
+    Elem<L> pw = rmod;
+    for (int k = 0; k < MAX_POINTS; ++k) {
+      for (int i = 0; i < L; ++i) fc.pow32[k][i] = pw.v[i];
+      pw = dbl_times(pw, 5);
+    }
   }
   // x^(p-2) in the Montgomery domain (the Montgomery image of x^-1 for x != 0).
   static Elem<L> mont_inverse(const Elem<L>& x, const Elem<L>& one_mont) {
@@ -419,6 +424,12 @@ uint32_t* buf_for_round(const sc_proof* p, int j) {
   return (j & 1) ? p->d_a : p->d_b;
 }
 
+// The pass kernel's round 0 of a compiled 255-bit composition leaves its sum scaled by
+// 32^-static_round0_scale (layout.hpp); the finalize restores it.
+int round0_scale_pow(const sc_proof* p, int j, bool wide) {
+  return (j == 0 && !wide && p->L == 8 && p->sid != 0) ? static_round0_scale(p->sid) : 0;
+}
+
 int enqueue_rounds(sc_proof* p, int mode) {
   hipStream_t s = p->ctx->stream;
   const FieldConsts* fc = p->ctx->d_fc[p->fidx];
@@ -473,6 +484,8 @@ int enqueue_rounds(sc_proof* p, int mode) {
     f.fc = fc;
     f.st = p->d_state;
     f.explicit_r = p->d_explicit;
+    f.scale_pow = round0_scale_pow(p, j, wide);
+    f.pad0 = 0;
     SC_HIP(p->ln->finalize(s, p->pmax, f));
   }
   if (jt < p->num_vars) {

@@ -30,6 +30,9 @@ template <class F>
 struct Work {
   static constexpr int L = F::L;
   static constexpr bool kMontIsStored = true;  // from_mont and from_stored read the same form
+  // The 255-bit form reads Montgomery input more cheaply in two places (Work<FieldP256>):
+  // folding straight from it, and reading it scaled in round 0 of compiled compositions.
+  static constexpr bool kCheapMontInput = false;
   using V = Elem<L>;
   SC_HD static V load(const uint32_t* w) {
     V x;
@@ -301,8 +304,39 @@ template <>
 struct Work<FieldP256> {
   static constexpr int L = 8;
   static constexpr bool kMontIsStored = false;
+  static constexpr bool kCheapMontInput = true;
   using V = R29;
   SC_HD static V from_mont(const uint32_t* w) { return r29_reduce(r29_unpack<5>(w)); }
+  // The caller's Montgomery X = x 2^256 < p taken as the working value itself, which stands
+  // for the residue x 2^-5 (no shift, no reduction; bounds as a stored value).
+  SC_HD static V from_mont_scaled(const uint32_t* w) { return r29_unpack<0>(w); }
+  // 32 r for a working r below p + 2^241: five doublings, normalized (value < 32.1 p).
+  SC_HD static V times32(V x) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) x.l[i] += x.l[i];
+      x = r29_normalize(x);
+    }
+    return x;
+  }
+  // Fold of two Montgomery input entries lo_w, hi_w with challenge r, given r32 = times32(r):
+  // 32 X_lo + (32 r)(X_hi - X_lo) 2^-261 is the working value of x_lo + r (x_hi - x_lo), so
+  // the inputs need no reduction of their own. X_hi + 3p - X_lo < 4p; the product is below
+  // 32.1p * 4p / 2^261 + p < 3p, and the sum below 35p (top limb < 2^29), which the final
+  // reduction brings below p + 2^241.
+  SC_HD static V fold_mont(const uint32_t* lo_w, const uint32_t* hi_w, const V& r32) {
+    const R29 lo = r29_unpack<0>(lo_w), hi = r29_unpack<0>(hi_w);
+    R29 d;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) d.l[i] = hi.l[i] + r29::PAD(3, i) - lo.l[i];
+    const R29 m = r29_mul(r32, d);
+    const R29 lo32 = r29_unpack<5>(lo_w);
+    R29 sum;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) sum.l[i] = lo32.l[i] + m.l[i];
+    return r29_reduce(sum);
+  }
   SC_HD static V from_stored(const uint32_t* w) { return r29_unpack<0>(w); }
   SC_HD static V coeff(const uint32_t* w) { return r29_unpack<0>(w); }
   SC_HD static void store(uint32_t* w, const V& x) { r29_pack<0>(w, x); }
@@ -493,6 +527,7 @@ template <>
 struct Work<FieldP128> {
   static constexpr int L = 4;
   static constexpr bool kMontIsStored = false;
+  static constexpr bool kCheapMontInput = false;
   using V = R26;
   // Montgomery X = 2x (X < p): x is X rotated right by one within 127 bits.
   SC_HD static V from_mont(const uint32_t* w) {

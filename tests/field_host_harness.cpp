@@ -44,6 +44,9 @@ extern "C" void field_host_op(int bit_width, int op, uint64_t n, const uint32_t*
 //   op 5: a x^2 + c a, x = a + 7 (b - a), the last two products sharing one reduction
 //   op 6: -x(t) into a fresh accumulator, t = i mod 8, x(t) reached by interpolation steps
 //   op 7: -x(t) into a fresh accumulator, t = i mod 8, x(t) = at(a, b - a, t)
+//   op 8: a + c (b - a) folded straight from the Montgomery words of a and b (fold_mont),
+//         through store and reload (fields with a cheaper Montgomery-input path; else op 1)
+//   op 9: a read scaled (from_mont_scaled: the residue times 2^-5; else from_mont)
 template <class F>
 static void run_work(int op, uint64_t n, const uint32_t* a, const uint32_t* b, const uint32_t* c, uint32_t* out) {
   using W = Work<F>;
@@ -72,6 +75,20 @@ static void run_work(int op, uint64_t n, const uint32_t* a, const uint32_t* b, c
       typename W::V t = y;
       for (int k = 2; k <= 7; ++k) t = W::step(t, d);
       acc.add(W::mul_sum(W::mul(x, t), t, z, x));  // a x7^2 + c a with x7 = a + 7 (b - a)
+    } else if (op == 8) {
+      uint32_t w[L];
+      if constexpr (W::kCheapMontInput) {
+        W::store(w, W::fold_mont(a + i * L, b + i * L, W::times32(z)));
+      } else {
+        W::store(w, W::fold(x, y, z));
+      }
+      acc.add(W::from_stored(w));
+    } else if (op == 9) {
+      if constexpr (W::kCheapMontInput) {
+        acc.add(W::from_mont_scaled(a + i * L));
+      } else {
+        acc.add(x);
+      }
     } else if (op == 6 || op == 7) {
       // a single-factor term with coefficient -1 is the first term a thread adds: the
       // subtraction must hold for the largest interpolated value on its own

@@ -145,6 +145,10 @@ def test_host_working_form(harness, bw):
     want6 = [(-(x + (i % 8) * (y - x))) % p for i, (x, y) in enumerate(zip(a, b))]
     assert harness.work(bw, 6, a, b, c) == want6
     assert harness.work(bw, 7, a, b, c) == want6
+    # round 1 folding straight from the caller's Montgomery words, and round 0's scaled read
+    assert harness.work(bw, 8, a, b, c) == [(x + mont(z, y - x)) % p for x, y, z in zip(a, b, c)]
+    inv32 = pow(32, -1, p) if bw == 256 else 1
+    assert harness.work(bw, 9, a, b, c) == [x * inv32 % p for x in a]
 
 
 def test_host_goldilocks_product_sum_one_reduction(harness):
