@@ -830,12 +830,75 @@ __device__ __forceinline__ void wide_fold_range(const uint32_t* __restrict__ src
   }
 }
 
-// Point-parallel evaluation of pairs begin + threadIdx.x / G, then every stride pairs, below
-// end: lane t = threadIdx.x % G evaluates the composition at point t only. Writes the block's
-// sums for t < npts to part[t * L + k].
-template <class F, int TMAX, int G>
+// x[v] for a wave-uniform table slot v, with x in registers: a switch, so a scalar branch to
+// the slot's copy (an indexed read of a register array would go through scratch memory; an
+// if-chain over the slots is folded back into such a read by the compiler, a switch is not).
+#define SC_SLOT_CASE(I, STMT) \
+  case I:                     \
+    if constexpr (I < N) {    \
+      constexpr int kSlot = I; \
+      STMT;                   \
+    }                         \
+    break;
+#define SC_SLOT_SWITCH(V, STMT)                                                                 \
+  switch (V) {                                                                                  \
+    SC_SLOT_CASE(0, STMT) SC_SLOT_CASE(1, STMT) SC_SLOT_CASE(2, STMT) SC_SLOT_CASE(3, STMT)       \
+    SC_SLOT_CASE(4, STMT) SC_SLOT_CASE(5, STMT) SC_SLOT_CASE(6, STMT) SC_SLOT_CASE(7, STMT)       \
+    SC_SLOT_CASE(8, STMT) SC_SLOT_CASE(9, STMT) SC_SLOT_CASE(10, STMT) SC_SLOT_CASE(11, STMT)     \
+    SC_SLOT_CASE(12, STMT) SC_SLOT_CASE(13, STMT) SC_SLOT_CASE(14, STMT) SC_SLOT_CASE(15, STMT)   \
+    SC_SLOT_CASE(16, STMT) SC_SLOT_CASE(17, STMT) SC_SLOT_CASE(18, STMT) SC_SLOT_CASE(19, STMT)   \
+    SC_SLOT_CASE(20, STMT) SC_SLOT_CASE(21, STMT) SC_SLOT_CASE(22, STMT) SC_SLOT_CASE(23, STMT)   \
+    SC_SLOT_CASE(24, STMT) SC_SLOT_CASE(25, STMT) SC_SLOT_CASE(26, STMT) SC_SLOT_CASE(27, STMT)   \
+    SC_SLOT_CASE(28, STMT) SC_SLOT_CASE(29, STMT) SC_SLOT_CASE(30, STMT) SC_SLOT_CASE(31, STMT)   \
+    default:                                                                                    \
+      break;                                                                                    \
+  }
+static_assert(MAX_TABLES == 32, "one switch case per table slot");
+
+template <class F, int N>
+__device__ __forceinline__ void mul_slot(typename Work<F>::V& prod, const typename Work<F>::V (&x)[N], int v) {
+  SC_SLOT_SWITCH(v, prod = Work<F>::mul(prod, x[kSlot]))
+}
+
+template <class F, int N>
+__device__ __forceinline__ typename Work<F>::V pick_slot(const typename Work<F>::V (&x)[N], int v) {
+  typename Work<F>::V out = x[0];
+  SC_SLOT_SWITCH(v, out = x[kSlot])
+  return out;
+}
+
+// C(x) by the factor lists of the descriptor: every table index a term names is wave-uniform,
+// so each factor is a scalar branch to a multiplication by a fixed register slot. A general
+// coefficient starts the product (its multiplication is one of the term's len); a +-1 term
+// starts from its first factor.
+template <class F, int N>
+__device__ __forceinline__ void compose_slots(typename Work<F>::Acc& acc, const typename Work<F>::V (&x)[N],
+                                              const CompDesc* __restrict__ cd, int K) {
+  using W = Work<F>;
+  for (int k = 0; k < K; ++k) {
+    const int len = cd->len[k];
+    const int kind = cd->kind[k];
+    typename W::V prod;
+    int m = 0;
+    if (kind == COEFF_GENERAL) {
+      prod = W::coeff(cd->coeff[k]);
+    } else {
+      prod = pick_slot<F, N>(x, cd->fac[k][0]);
+      m = 1;
+    }
+    for (; m < len; ++m) mul_slot<F, N>(prod, x, cd->fac[k][m]);
+    if (kind == COEFF_MINUS_ONE) acc.sub(prod); else acc.add(prod);
+  }
+}
+
+// Point-parallel evaluation of pairs [begin, end) of round `round`: lane q of a group of
+// P = wide_points(npts, round) lanes evaluates the composition at the q-th evaluated point of
+// pair begin + threadIdx.x / P, then every stride pairs (lanes past the last whole group
+// idle). Writes the block's sums for t < npts to part[t * L + k] (zero for t = 1 from round
+// 1 on, which the round's finish derives from the claim). s_red: BLOCK * L words.
+template <class F, int TMAX>
 __device__ __forceinline__ void wide_eval_block(const uint32_t* __restrict__ src, uint64_t src_n, bool src_mont,
-                                                uint64_t begin, uint64_t end, uint64_t stride,
+                                                int round, uint64_t begin, uint64_t end, uint64_t stride,
                                                 const CompDesc* __restrict__ cd, uint32_t* __restrict__ part,
                                                 uint32_t* s_red) {
   using W = Work<F>;
@@ -843,48 +906,60 @@ __device__ __forceinline__ void wide_eval_block(const uint32_t* __restrict__ src
   const int T = cd->num_tables;
   const int K = cd->num_terms;
   const int npts = cd->npts;
-  const int t = threadIdx.x % G;
+  const int P = wide_points(npts, round);
+  const int pairs_here = BLOCK / P;
+  const int q = threadIdx.x % P;
+  const int group = threadIdx.x / P;
+  const int t = (round > 0 && q >= 1) ? q + 1 : q;
   typename W::Acc acc;
   acc.clear();
-  for (uint64_t i = begin + threadIdx.x / G; i < end; i += stride) {
-    typename W::V x[TMAX];
-    static_for<TMAX>([&](auto vi) {
-      constexpr int v = decltype(vi)::value;
-      if (v < T) {
-        uint32_t w[2 * L];
-        load_words<2 * L>(w, src + ((uint64_t)v * src_n + 2 * i) * L);
-        const typename W::V lo = src_mont ? W::from_mont(w) : W::from_stored(w);
-        const typename W::V hi = src_mont ? W::from_mont(w + L) : W::from_stored(w + L);
-        x[v] = W::at(lo, W::diff(hi, lo), t);
+  if (group < pairs_here) {
+    for (uint64_t i = begin + group; i < end; i += stride) {
+      typename W::V x[TMAX];
+      static_for<TMAX>([&](auto vi) {
+        constexpr int v = decltype(vi)::value;
+        if (v < T) {
+          uint32_t w[2 * L];
+          load_words<2 * L>(w, src + ((uint64_t)v * src_n + 2 * i) * L);
+          const typename W::V lo = src_mont ? W::from_mont(w) : W::from_stored(w);
+          const typename W::V hi = src_mont ? W::from_mont(w + L) : W::from_stored(w + L);
+          x[v] = W::at(lo, W::diff(hi, lo), t);
+        } else {
+          x[v] = x[0];  // defined (an undefined slot measured 50 more VGPRs), never read
+        }
+      });
+      // Register slots while TMAX values fit in 160 VGPRs; 32 values of the 255-bit field do not,
+      // and keep the term loop (x then lives in scratch memory).
+      constexpr bool kSlots = TMAX > 8 && TMAX * (int)(sizeof(typename W::V) / 4) <= 160;
+      if constexpr (kSlots) {
+        compose_slots<F, TMAX>(acc, x, cd, K);
       } else {
-        x[v] = x[0];
+        compose_add<F, TMAX>(acc, x, cd, K);
       }
-    });
-    // The term loop over TMAX slots stays a loop here (x in scratch, the rest in registers):
-    // expanding it needs 1 wave per SIMD and measured slower.
-    if (t < npts) compose_add<F, TMAX>(acc, x, cd, K);
+    }
   }
-  // Lanes whose index differs by a multiple of G hold the same point.
-  Elem<L> sum = acc.value();
+  // Sum each point's lanes: lane values to LDS, then per point a tree over its groups.
+  const Elem<L> mine = acc.value();
 #pragma unroll
-  for (int off = G; off < 64; off <<= 1) {
-    Elem<L> o;
-#pragma unroll
-    for (int k = 0; k < L; ++k) o.v[k] = __shfl_xor(sum.v[k], off, 64);
-    sum = add<F>(sum, o);
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane < G) {
-#pragma unroll
-    for (int k = 0; k < L; ++k) s_red[(wave * G + lane) * L + k] = sum.v[k];
-  }
+  for (int k = 0; k < L; ++k) s_red[threadIdx.x * L + k] = group < pairs_here ? mine.v[k] : 0u;
   __syncthreads();
+  for (int half = 128; half >= 1; half >>= 1) {
+    // lane (g, q) with g < half adds group g + half of the same point
+    if (group < half && group + half < pairs_here) {
+      const Elem<L> a = load_elem<L>(s_red + threadIdx.x * L);
+      const Elem<L> b = load_elem<L>(s_red + (threadIdx.x + half * P) * L);
+      const Elem<L> c = add<F>(a, b);
+#pragma unroll
+      for (int k = 0; k < L; ++k) s_red[threadIdx.x * L + k] = c.v[k];
+    }
+    __syncthreads();
+  }
   if (threadIdx.x < npts) {
-    Elem<L> s2 = load_elem<L>(s_red + threadIdx.x * L);
+    const int tt = threadIdx.x;
+    const int qq = (round > 0 && tt >= 1) ? tt - 1 : tt;  // lane of point tt (tt = 1 skipped)
+    const bool skipped = round > 0 && tt == 1;
 #pragma unroll
-    for (int w = 1; w < BLOCK / 64; ++w) s2 = add<F>(s2, load_elem<L>(s_red + (w * G + threadIdx.x) * L));
-#pragma unroll
-    for (int k = 0; k < L; ++k) part[threadIdx.x * L + k] = s2.v[k];
+    for (int k = 0; k < L; ++k) part[tt * L + k] = skipped ? 0u : s_red[qq * L + k];
   }
 }
 
@@ -897,11 +972,11 @@ __global__ void __launch_bounds__(BLOCK) wide_fold_kernel(PassArgs a) {
 
 template <class F, int TMAX, int G>
 __global__ void __launch_bounds__(BLOCK) wide_eval_kernel(PassArgs a) {
-  constexpr int PAIRS = BLOCK / G;  // pairs per block per sweep
-  __shared__ uint32_t s_red[(BLOCK / 64) * G * F::L];
-  wide_eval_block<F, TMAX, G>(a.src, a.src_n, a.round == 0, (uint64_t)blockIdx.x * PAIRS, a.pairs,
-                              (uint64_t)gridDim.x * PAIRS, a.comp,
-                              a.partials + (uint64_t)blockIdx.x * MAX_POINTS * F::L, s_red);
+  __shared__ uint32_t s_red[BLOCK * F::L];
+  const uint64_t pairs_here = wide_pairs_per_block(a.comp->npts, a.round);
+  wide_eval_block<F, TMAX>(a.src, a.src_n, a.round == 0, a.round, (uint64_t)blockIdx.x * pairs_here, a.pairs,
+                           (uint64_t)gridDim.x * pairs_here, a.comp,
+                           a.partials + (uint64_t)blockIdx.x * MAX_POINTS * F::L, s_red);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -985,18 +1060,18 @@ __global__ void __launch_bounds__(BLOCK) batch_wide_fold_kernel(BatchArgs a) {
 // (BLOCK / G) * items pairs of its entry's round and writes the tile's partial.
 template <class F, int TMAX, int G>
 __global__ void __launch_bounds__(BLOCK) batch_wide_eval_kernel(BatchArgs a) {
-  constexpr int PAIRS = BLOCK / G;
-  __shared__ uint32_t s_red[(BLOCK / 64) * G * F::L];
+  __shared__ uint32_t s_red[BLOCK * F::L];
   const uint32_t b = blockIdx.x + a.tile_base;
   const StageEntry ent = a.entries[find_entry(a.entries, a.n_entries, b)];
   const InstDesc& in = a.insts[ent.inst];
   const int j = (int)ent.round;
   const uint32_t* src = j == 0 ? in.input : ((j & 1) ? in.buf_a : in.buf_b);
   const uint64_t pairs = in.n0 >> (j + 1);
-  const uint64_t span = (uint64_t)PAIRS * a.items;
+  const uint64_t pairs_here = wide_pairs_per_block(in.comp->npts, j);
+  const uint64_t span = pairs_here * a.items;
   const uint64_t begin = (uint64_t)(b - ent.tile_begin) * span;
-  wide_eval_block<F, TMAX, G>(src, in.n0 >> j, j == 0, begin, begin + span < pairs ? begin + span : pairs, PAIRS,
-                              in.comp, a.partials + (uint64_t)b * (MAX_POINTS * MAX_LIMBS), s_red);
+  wide_eval_block<F, TMAX>(src, in.n0 >> j, j == 0, j, begin, begin + span < pairs ? begin + span : pairs,
+                           pairs_here, in.comp, a.partials + (uint64_t)b * (MAX_POINTS * MAX_LIMBS), s_red);
 }
 
 // One block per entry: the entry's instance runs all its remaining rounds from entry.round in

@@ -92,7 +92,15 @@ struct CompDesc {
   uint32_t exps[MAX_TERMS][MAX_TABLES / 8];  // 4-bit exponent of table v in term k
   int32_t kind[MAX_TERMS];
   uint32_t coeff[MAX_TERMS][MAX_LIMBS];  // Montgomery form, used when kind == COEFF_GENERAL
+  // The same terms as factor lists: term k is the product of tables fac[k][0 .. len[k]).
+  int32_t len[MAX_TERMS];
+  uint8_t fac[MAX_TERMS][MAX_POINTS];
 };
+
+// Point-parallel evaluation: one lane per (pair, evaluated point). Round 0 evaluates all
+// npts points; later rounds skip t = 1, whose value the round's finish derives from the claim.
+constexpr int wide_points(int npts, int round) { return round > 0 ? npts - 1 : npts; }
+constexpr uint32_t wide_pairs_per_block(int npts, int round) { return BLOCK / wide_points(npts, round); }
 
 struct FieldConsts {
   uint32_t one[MAX_LIMBS];               // R mod p, the Montgomery image of 1

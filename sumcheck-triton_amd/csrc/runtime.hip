@@ -171,6 +171,11 @@ int build_comp(const sc_composition* c, const FieldConsts& fc, CompDesc& d, int&
       if (v < 0 || v >= c->num_tables) return fail(SC_ERR_ARG, "term_vars entry out of range");
       ++counts[v];
     }
+    // factor list in table order (the product does not depend on the order)
+    d.len[k] = len;
+    int m = 0;
+    for (int v = 0; v < c->num_tables; ++v)
+      for (int e = 0; e < counts[v]; ++e) d.fac[k][m++] = (uint8_t)v;
     for (int v = 0; v < MAX_TABLES; ++v) d.exps[k][v >> 3] |= (uint32_t)counts[v] << ((v & 7) * 4);
     Elem<L> coeff = zero<F>();
     if (c->coeffs != nullptr) coeff = H::from(c->coeffs + (size_t)k * L);
@@ -338,13 +343,13 @@ struct StagePlan {
   uint64_t pass_tiles = 0, eval_tiles = 0, fold_tiles = 0;
 
   // Partial tiles the round adds: pass tiles of BLOCK * items pairs, or evaluation tiles of
-  // (BLOCK / pmax) * items pairs.
-  static uint64_t tiles(uint64_t pairs, bool wide, uint32_t items, int pmax) {
-    const uint64_t span = (wide ? (uint64_t)(BLOCK / pmax) : (uint64_t)BLOCK) * items;
+  // wide_pairs_per_block(npts, round) * items pairs.
+  static uint64_t tiles(uint64_t pairs, bool wide, uint32_t items, int npts, uint32_t round) {
+    const uint64_t span = (wide ? (uint64_t)wide_pairs_per_block(npts, (int)round) : (uint64_t)BLOCK) * items;
     return (pairs + span - 1) / span;
   }
-  void add(uint32_t inst, uint32_t round, uint64_t pairs, int num_tables, bool wide, uint32_t items, int pmax) {
-    const uint32_t t = (uint32_t)tiles(pairs, wide, items, pmax);
+  void add(uint32_t inst, uint32_t round, uint64_t pairs, int num_tables, bool wide, uint32_t items, int npts) {
+    const uint32_t t = (uint32_t)tiles(pairs, wide, items, npts, round);
     if (!wide) {
       pass.push_back(StageEntry{inst, round, (uint32_t)pass_tiles, t});
       pass_tiles += t;
@@ -466,7 +471,7 @@ int enqueue_rounds(sc_proof* p, int mode) {
       e.src_n = p->n0 >> j;
       e.fold = 0;
       e.fused_finalize = 0;
-      const uint64_t pairs_per_block = BLOCK / p->pmax;
+      const uint64_t pairs_per_block = wide_pairs_per_block(p->npts, j);
       g = std::min<uint64_t>((a.pairs + pairs_per_block - 1) / pairs_per_block, (uint64_t)MAX_PASS_BLOCKS);
       SC_HIP(p->ln->wide_eval(s, p->tmax, p->pmax, e, (unsigned)g));
     } else {
@@ -1203,7 +1208,7 @@ int build_stages(sc_batch* b) {
       if (s >= b->tail_round[i]) continue;
       const uint64_t pairs = b->n0[i] >> (s + 1);
       plan.add((uint32_t)i, (uint32_t)s, pairs, b->num_tables[i], (int64_t)pairs <= b->wide_pairs, b->items,
-               b->pmax);
+               b->npts[i]);
       if (plan.partial_tiles() > 0x7FFFFFFFull || plan.fold_tiles > 0x7FFFFFFFull)
         return fail(SC_ERR_UNSUPPORTED, "batch stage has too many tiles");
     }
@@ -1842,9 +1847,9 @@ int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
         continue;
       }
       const bool wide = (int64_t)pairs <= wide_pairs;
-      if (plan.partial_tiles() + StagePlan::tiles(pairs, wide, sv->items, pmax) > sv->max_tiles)
+      if (plan.partial_tiles() + StagePlan::tiles(pairs, wide, sv->items, r.npts, (uint32_t)r.round) > sv->max_tiles)
         continue;  // runs in a later step
-      plan.add((uint32_t)slot, (uint32_t)r.round, pairs, r.desc.num_tables, wide, sv->items, pmax);
+      plan.add((uint32_t)slot, (uint32_t)r.round, pairs, r.desc.num_tables, wide, sv->items, r.npts);
     }
     if (plan.size() == 0 && tails.empty()) continue;
     std::vector<StageEntry> ents;
