@@ -174,16 +174,20 @@ def measure_plain(ctx, bw: int, comp, nv: int, reps: int, selector_mask: int = 0
 
 def measure_batched(ctx, bw: int, n: int = 256, reps: int = 3) -> dict:
     """Batched mix: n uneven instances (num_vars 10..22, random product sums over 2..8 tables),
-    proved one at a time (plain) and as one batch; both SHA3. The tile size of the batch is
-    picked by timing candidates (autotune) before the timed runs; results must agree."""
+    proved one at a time (plain) and as one batch; both SHA3. The batch's schedule is tuned
+    per (width, degree, size) class of instance (autotune.tune_batch) before the timed runs.
+    Both sides are replayed as captured hipGraphs (each plain proof its own graph), and the
+    plain proofs are also timed eagerly. Results must agree."""
     from zkduel_hip import Proof
+    from zkduel_hip.autotune import tune_batch
     from zkduel_hip.prover import Batch
     from zkduel_hip.synthetic import batch_instances
 
     insts = batch_instances(n, seed=4)
     with Batch(ctx, bw, insts) as b:
-        tune = b.autotune((1, 2, 4, 8, 16, 32), "sha3")
-        best = min(tune, key=tune.get)
+        t0 = time.perf_counter()
+        tune = tune_batch(b, "sha3")
+        tune_s = time.perf_counter() - t0
         b.capture("sha3")
         b.run("sha3")
         b.wait()
@@ -196,7 +200,18 @@ def measure_batched(ctx, bw: int, n: int = 256, reps: int = 3) -> dict:
         info = b.info()
     proofs = [Proof(ctx, bw, c, nv, input="uniform", seed=inp["seed"], instance=inp["instance"]) for c, nv, inp in insts]
     try:
+        for pr in proofs:  # eager first
+            pr.run("sha3")
+            pr.wait()
+        ctx.sync()
+        t0 = time.perf_counter()
         for pr in proofs:
+            pr.run("sha3")
+            pr.wait()
+            pr.result()
+        eager_s = time.perf_counter() - t0
+        for pr in proofs:
+            pr.capture("sha3")
             pr.run("sha3")
             pr.wait()
         ctx.sync()
@@ -210,14 +225,18 @@ def measure_batched(ctx, bw: int, n: int = 256, reps: int = 3) -> dict:
     finally:
         for pr in proofs:
             pr.close()
-    return {"instances": n, "bit_width": bw, "tile_items_autotuned": best,
-            "tile_candidates_ms": {str(k): round(v, 3) for k, v in tune.items()},
+    return {"instances": n, "bit_width": bw,
+            "tuned_classes": tune["classes"], "tune_default_ms": round(tune["default_ms"], 3),
+            "tune_ms": round(tune["ms"], 3), "tune_s": round(tune_s, 2),
             "batched_proofs_per_s": n / batched_s, "plain_proofs_per_s": n / plain_s,
             "speedup": plain_s / batched_s, "batched_ms": batched_s * 1000.0, "plain_ms": plain_s * 1000.0,
+            "plain": "each proof replayed as its own captured hipGraph, one after another",
+            "plain_eager_ms": eager_s * 1000.0, "speedup_vs_eager_plain": eager_s / batched_s,
             "results_identical": batch_cs == plain_cs, "device_bytes": info["device_bytes"]}
 
 
-def measure_serving(ctx, rate_per_s: float, n: int, budget_bytes: int = 200 * 10**9) -> dict:
+def measure_serving(ctx, rate_per_s: float, n: int, budget_bytes: int = 200 * 10**9,
+                    schedules: dict | None = None) -> dict:
     """Serving: n requests with Poisson arrivals at rate_per_s (num_vars log-uniform 12..24,
     widths 256/64/32 at 0.5/0.3/0.2, random product sums) replayed in real time against the
     continuous-batching server under an HBM budget; every checksum is then checked against
@@ -227,7 +246,7 @@ def measure_serving(ctx, rate_per_s: float, n: int, budget_bytes: int = 200 * 10
     from zkduel_hip.synthetic import serving_requests
 
     reqs = serving_requests(n, rate_per_s=rate_per_s, seed=5)
-    st = run_poisson(ctx, reqs, hbm_budget_bytes=budget_bytes, max_active=256)
+    st = run_poisson(ctx, reqs, hbm_budget_bytes=budget_bytes, max_active=256, schedules=schedules)
     same = True
     for r in reqs[:: max(1, n // 64)]:  # a spread sample of requests, each re-proved alone
         with Proof(ctx, r["bit_width"], r["composition"], r["num_vars"], input="uniform",
@@ -378,9 +397,11 @@ def main() -> int:
             log(f"bench:   {batched[str(bw)]['batched_proofs_per_s']:.0f} vs {batched[str(bw)]['plain_proofs_per_s']:.0f} proofs/s")
     serving = {}
     if not args.no_batch:
+        # the batched mix's tuned class schedules, applied to served requests of those classes
+        schedules = {c: v for r in batched.values() for c, v in r["tuned_classes"].items()}
         for rate, n in ((300.0, 600), (5000.0, 600)):
             log(f"bench: serving, Poisson arrivals at {rate:.0f}/s, {n} requests")
-            serving[f"rate_{rate:.0f}_per_s"] = measure_serving(ctx, rate, n)
+            serving[f"rate_{rate:.0f}_per_s"] = measure_serving(ctx, rate, n, schedules=schedules)
             r = serving[f"rate_{rate:.0f}_per_s"]
             log(f"bench:   {r['proofs_per_s']:.0f} proofs/s, p50 {r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms")
     cpu = None

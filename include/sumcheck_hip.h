@@ -94,12 +94,13 @@ int sc_proof_set_schedule(sc_proof* proof, int32_t max_pass_blocks, int64_t tail
                           int32_t fused_finalize);
 
 /* Kernel choice per round: rounds with at most max_pairs pairs run as a fold launch plus the
- * point-parallel evaluation (one lane per pair and point) instead of the per-pair pass kernel
- * with the fold fused in; 0 never, a negative value restores the default (every round for
- * interpreted compositions of more than 8 tables, rounds below 2^20 pairs for interpreted and
- * below 2^18 for compiled 255-bit compositions, none otherwise). Results do not depend on it. No counterpart in the
- * reference, which runs every round as an eval, a reduce and a separate fold kernel
- * (field_sumcheck.cu:614-694). */
+ * point-parallel evaluation (one lane per pair and evaluated point) instead of the per-pair
+ * pass kernel with the fold fused in; 0 never, a negative value restores the default (every
+ * round for interpreted compositions of more than 8 tables and for interpreted 255-bit
+ * compositions of degree 3 or more, rounds below 2^20 pairs for other interpreted and below
+ * 2^18 for compiled 255-bit compositions, none otherwise). Results do not depend on it. No
+ * counterpart in the reference, which runs every round as an eval, a reduce and a separate
+ * fold kernel (field_sumcheck.cu:614-694). */
 int sc_proof_set_point_parallel(sc_proof* proof, int64_t max_pairs);
 
 /* Enqueues one whole proof (all rounds, transcript on the device) on the context's stream
@@ -196,15 +197,29 @@ int sc_transcript_challenge(sc_context* ctx, int32_t bit_width, int32_t round, c
 typedef struct sc_batch sc_batch;
 int sc_batch_create(sc_context* ctx, int32_t bit_width, int32_t n, const sc_composition* comps,
                     const int32_t* num_vars, const sc_input* inputs, sc_batch** out);
-/* Pairs per thread in a tile (tile = 256 * items pairs; 256 / (degree + 1 rounded up to 4 or
- * 8) * items pairs in point-parallel rounds); default 4. */
+/* Pairs per thread in a tile of every instance (tile = 256 * items pairs; 256 / (points
+ * evaluated per pair: degree + 1 in round 0, degree later) * items pairs in point-parallel
+ * rounds); default 4. */
 int sc_batch_set_tile(sc_batch* batch, int32_t items_per_thread);
 /* Instance rounds with at most max_pairs pairs run point-parallel (a fold launch plus an
  * evaluation launch with one lane per pair and point, as sc_proof_set_point_parallel for one
- * proof); 0 never, negative the default (every round when an instance of the batch has more
- * than 8 tables, none otherwise). Results do not depend on it.
+ * proof), for every instance; 0 never, negative each instance's default (see
+ * sc_batch_set_schedules). Results do not depend on it.
  * No counterpart in the reference (one instance per run). */
 int sc_batch_set_point_parallel(sc_batch* batch, int64_t max_pairs);
+/* Per-instance schedules (no counterpart in the reference), arrays of the batch's size, any
+ * of them NULL to keep the current values: items_per_thread[i] the instance's tile ([1, 64];
+ * 0 keeps it), max_pairs[i] the pair count at or below which its rounds run point-parallel
+ * (negative: the default, every round when the batch holds an instance of more than 8 tables
+ * or, at 256 bits, when the instance's degree is 3 or more; otherwise never), tail_pairs[i]
+ * the pair count at or below which its remaining rounds run in the batch's tail launch (0:
+ * the default, 512 pairs, 64 at 256 bits, 1 with more than 8 tables). Results do not depend
+ * on them. sc_batch_instance_schedule reads back one instance's tile, the point-parallel
+ * threshold in effect and the first round of its tail. */
+int sc_batch_set_schedules(sc_batch* batch, const int32_t* items_per_thread, const int64_t* max_pairs,
+                           const int64_t* tail_pairs);
+int sc_batch_instance_schedule(sc_batch* batch, int32_t instance, int32_t* items_per_thread,
+                               int64_t* max_pairs, int32_t* tail_round);
 /* challenge_mode SC_CHALLENGE_FIXED or SC_CHALLENGE_SHA3; enqueues without waiting. */
 int sc_batch_run(sc_batch* batch, int32_t challenge_mode);
 int sc_batch_capture(sc_batch* batch, int32_t challenge_mode);
@@ -216,18 +231,28 @@ int sc_batch_destroy(sc_batch* batch);
 
 /* Serving (no counterpart in the reference): continuous batching of proof requests. Requests
  * are queued by sc_server_submit and admitted at step boundaries, first in first out, while a
- * slot (max_active) and HBM (hbm_budget_bytes, the arena holding their tables: 1.75x input)
- * are free. Each sc_server_step enqueues one round of every running request, each at its own
- * round, using the batched stage kernels grouped by width; tables and transcript state stay
- * on the device between steps. A request's result is the same bits as sc_prove on it.
- * Results are read once (sc_server_result frees the request). Poll states: 0 queued,
- * 1 running, 2 done. Calls on one server must be serialized. */
+ * slot (max_active) and HBM (hbm_budget_bytes: their tables take 1.75x the input) are free.
+ * The server's device memory is allocated on demand, in chunks of doubling size from 4 GiB up
+ * to the budget (a request larger than the next chunk gets one of its own, released when it
+ * finishes), so creation allocates no table memory. Each sc_server_step enqueues one round of
+ * every running request, each at its own round, oldest first, using the batched stage kernels
+ * grouped by width; tables and transcript state stay on the device between steps. A request's
+ * result is the same bits as sc_prove on it. Results are read once (sc_server_result frees
+ * the request). Poll states: 0 queued, 1 running, 2 done. sc_server_info: arena_used = bytes
+ * of the admitted requests, arena_size = device bytes the server holds. Calls on one server
+ * must be serialized. */
 typedef struct sc_server sc_server;
 int sc_server_create(sc_context* ctx, uint64_t hbm_budget_bytes, int32_t max_active, sc_server** out);
 int sc_server_submit(sc_server* server, int32_t bit_width, const sc_composition* comp, int32_t num_vars,
                      const sc_input* input, int32_t challenge_mode, int64_t* handle);
 int sc_server_step(sc_server* server, int32_t* n_running, int32_t* n_queued);
 int sc_server_poll(sc_server* server, int64_t handle, int32_t* state);
+/* Schedule of a queued or running request, as sc_batch_set_schedules for one instance (tile and
+ * point-parallel threshold; the tail threshold is the default). A request's largest round
+ * must fit one step's partials (2^20 tiles): submit and this call raise its tile to fit, and
+ * refuse a request that cannot (SC_ERR_UNSUPPORTED). No counterpart in the reference. */
+int sc_server_set_request_schedule(sc_server* server, int64_t handle, int32_t items_per_thread,
+                                   int64_t max_pairs);
 int sc_server_drain(sc_server* server);
 int sc_server_result(sc_server* server, int64_t handle, uint32_t* round_polys, uint32_t* challenges,
                      uint64_t* checksum);

@@ -1020,8 +1020,8 @@ __global__ void __launch_bounds__(BLOCK) batch_pass_kernel(BatchArgs a) {
 #pragma unroll
   for (int t = 0; t < PMAX; ++t) acc[t].clear();
   const uint64_t tile = blockIdx.x - ent.tile_begin;
-  const uint64_t base = tile * (uint64_t)BLOCK * a.items;
-  for (uint32_t it = 0; it < a.items; ++it) {
+  const uint64_t base = tile * (uint64_t)BLOCK * ent.items;
+  for (uint32_t it = 0; it < ent.items; ++it) {
     const uint64_t i = base + (uint64_t)it * BLOCK + threadIdx.x;
     if (i < pairs) {
       typename W::V cur[TMAX], oth[TMAX];
@@ -1050,14 +1050,14 @@ __global__ void __launch_bounds__(BLOCK) batch_wide_fold_kernel(BatchArgs a) {
   uint32_t* dst = (j & 1) ? in.buf_a : in.buf_b;
   const uint64_t src_n = in.n0 >> (j - 1);
   const uint64_t total = (src_n >> 1) * (uint64_t)in.comp->num_tables;
-  const uint64_t span = (uint64_t)BLOCK * a.items;
+  const uint64_t span = (uint64_t)BLOCK * ent.items;
   const uint64_t begin = (uint64_t)(blockIdx.x - ent.tile_begin) * span;
   wide_fold_range<F>(src, dst, src_n, j <= 1, Work<F>::from_mont(in.st->r[j - 1]), begin + threadIdx.x,
                      begin + span < total ? begin + span : total, BLOCK);
 }
 
 // Evaluation: block b (tile index b + tile_base in the stage's partials) evaluates
-// (BLOCK / G) * items pairs of its entry's round and writes the tile's partial.
+// wide_pairs_per_block(npts, round) * items pairs of its entry's round and writes the tile's partial.
 template <class F, int TMAX, int G>
 __global__ void __launch_bounds__(BLOCK) batch_wide_eval_kernel(BatchArgs a) {
   __shared__ uint32_t s_red[BLOCK * F::L];
@@ -1068,7 +1068,7 @@ __global__ void __launch_bounds__(BLOCK) batch_wide_eval_kernel(BatchArgs a) {
   const uint32_t* src = j == 0 ? in.input : ((j & 1) ? in.buf_a : in.buf_b);
   const uint64_t pairs = in.n0 >> (j + 1);
   const uint64_t pairs_here = wide_pairs_per_block(in.comp->npts, j);
-  const uint64_t span = pairs_here * a.items;
+  const uint64_t span = pairs_here * ent.items;
   const uint64_t begin = (uint64_t)(b - ent.tile_begin) * span;
   wide_eval_block<F, TMAX>(src, in.n0 >> j, j == 0, j, begin, begin + span < pairs ? begin + span : pairs,
                            pairs_here, in.comp, a.partials + (uint64_t)b * (MAX_POINTS * MAX_LIMBS), s_red);

@@ -208,13 +208,14 @@ struct StageEntry {
   uint32_t round;
   uint32_t tile_begin;  // first tile (block) of this entry in the stage
   uint32_t tiles;
+  uint32_t items;       // pairs per thread of this entry's tiles (per-pair: BLOCK * items pairs)
 };
 
 struct BatchArgs {
   const InstDesc* insts;
   const StageEntry* entries;  // sorted by tile_begin
   uint32_t n_entries;
-  uint32_t items;             // pairs per thread in a tile: a tile is BLOCK * items pairs
+  uint32_t pad1;
   uint32_t* partials;         // [tiles][MAX_POINTS][MAX_LIMBS]
   const FieldConsts* fc;
   uint32_t tile_base;         // point-parallel evaluation: block b is tile b + tile_base

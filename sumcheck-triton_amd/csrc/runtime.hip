@@ -324,15 +324,22 @@ int64_t default_proof_tail_pairs(int L, bool over8) {
 int64_t default_wide_pairs(const sc_proof* p) {
   if (p->sid != 0) return p->L == 8 ? (int64_t(1) << 17) : 0;
   if (p->tmax > 8) return INT64_MAX;
-  return p->L == 8 ? (int64_t(1) << 19) : 0;
+  if (p->L != 8) return 0;
+  // 255-bit interpreted compositions of degree >= 3: every round (num_vars 24, MI355X: 5.94
+  // against 6.10 ms for a*b*c + d*e forced interpreted, 15.1 against 18.2 ms for a degree-5
+  // product sum); degree 2 keeps the per-pair kernel for rounds of 2^20 pairs and more
+  // (8 tables: 5.49 ms against 6.23 all point-parallel).
+  return p->npts >= 4 ? INT64_MAX : (int64_t(1) << 19);
 }
 
-// Batches and servers: point-parallel rounds only where the per-pair kernels would spill
-// (more than 8 tables). A stage's many instances already fill the GPU, which is what the
-// point-parallel split buys a single small proof; at 255 bits, 256 instances of num_vars
-// 10-22 (MI355X): 101 ms per-pair, 112 ms with rounds below 2^20 pairs point-parallel,
-// 118 ms all point-parallel.
-int64_t default_batch_wide_pairs(int tmax) { return tmax > 8 ? INT64_MAX : 0; }
+// Batches and servers, per instance: every round point-parallel where the per-pair kernels
+// would spill (the launch's table capacity above 8), and at 255 bits for compositions of
+// degree >= 3, whose per-pair kernels run one wave per SIMD (256 instances of num_vars 10-22,
+// MI355X: 81.9 ms all point-parallel against 100.8 ms per-pair); otherwise per-pair (64 bits:
+// 12.8 ms per-pair against 15.5 ms all point-parallel).
+int64_t default_instance_wide_pairs(int L, int tmax, int npts) {
+  return (tmax > 8 || (L == 8 && npts >= 4)) ? INT64_MAX : 0;
+}
 
 // Work lists of one batched stage (one round of each listed instance). Per-pair entries own
 // tiles [0, pass_tiles) of the stage's partials and point-parallel entries the next
@@ -351,16 +358,16 @@ struct StagePlan {
   void add(uint32_t inst, uint32_t round, uint64_t pairs, int num_tables, bool wide, uint32_t items, int npts) {
     const uint32_t t = (uint32_t)tiles(pairs, wide, items, npts, round);
     if (!wide) {
-      pass.push_back(StageEntry{inst, round, (uint32_t)pass_tiles, t});
+      pass.push_back(StageEntry{inst, round, (uint32_t)pass_tiles, t, items});
       pass_tiles += t;
       return;
     }
-    eval.push_back(StageEntry{inst, round, (uint32_t)eval_tiles, t});  // rebased in layout()
+    eval.push_back(StageEntry{inst, round, (uint32_t)eval_tiles, t, items});  // rebased in layout()
     eval_tiles += t;
     if (round > 0) {
       const uint64_t elems = 2 * pairs * (uint64_t)num_tables, span = (uint64_t)BLOCK * items;
       const uint32_t f = (uint32_t)((elems + span - 1) / span);
-      fold.push_back(StageEntry{inst, round, (uint32_t)fold_tiles, f});
+      fold.push_back(StageEntry{inst, round, (uint32_t)fold_tiles, f, items});
       fold_tiles += f;
     }
   }
@@ -1145,7 +1152,10 @@ struct sc_batch {
   // Stage s: its lists start at stage_offset[s] in d_entries, laid out as StagePlan::layout.
   std::vector<uint32_t> stage_offset, stage_pass, stage_eval, stage_fold;
   std::vector<uint64_t> stage_pass_tiles, stage_eval_tiles, stage_fold_tiles;
-  int64_t wide_pairs = 0;  // rounds with at most this many pairs run point-parallel
+  // Per-instance schedule: tile (pairs per thread), the pair count at or below which a round
+  // runs point-parallel (-1: default_instance_wide_pairs) and the tail threshold (0: default).
+  std::vector<uint32_t> items_i;
+  std::vector<int64_t> wide_i, tail_i;
   // Each instance's last rounds (from tail_round[i] on, at most tail_pairs pairs) run as one
   // tail entry of the last stage: one workgroup loops over them, as a single proof's tail
   // kernel does. All tails share that one launch (a tail launch per stage measured slower:
@@ -1153,7 +1163,6 @@ struct sc_batch {
   std::vector<uint32_t> tail_offset, tail_entries;
   std::vector<int> tail_round;
   int num_stages = 0;
-  uint32_t items = 4;
   int mode = -1;  // challenge mode baked into the instance descriptors
   std::vector<InstDesc> insts;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1185,8 +1194,24 @@ void drop_batch_graph(sc_batch* b) {
 }
 
 // Work lists of every stage for the current tile size, uploaded to the device.
+int64_t batch_wide_pairs(const sc_batch* b, int i) {
+  return b->wide_i[i] < 0 ? default_instance_wide_pairs(b->L, b->tmax, b->npts[i]) : b->wide_i[i];
+}
+
+// First round of instance i that runs in the batch's tail launch: the first whose pairs fit
+// its tail threshold (as for a single proof), at most its last round.
+void batch_tail_rounds(sc_batch* b) {
+  for (int i = 0; i < b->n; ++i) {
+    const int64_t tp = b->tail_i[i] > 0 ? b->tail_i[i] : default_tail_pairs(b->L, b->tmax > 8);
+    int j = 0;
+    while (j < b->num_vars[i] - 1 && (int64_t)(b->n0[i] >> (j + 1)) > tp) ++j;
+    b->tail_round[i] = j;
+  }
+}
+
 int build_stages(sc_batch* b) {
   std::vector<StageEntry> all;
+  batch_tail_rounds(b);
   b->num_stages = 0;
   for (int i = 0; i < b->n; ++i) b->num_stages = std::max(b->num_stages, b->tail_round[i] + 1);
   const int last = b->num_stages - 1;
@@ -1207,8 +1232,8 @@ int build_stages(sc_batch* b) {
     for (int i = 0; i < b->n; ++i) {
       if (s >= b->tail_round[i]) continue;
       const uint64_t pairs = b->n0[i] >> (s + 1);
-      plan.add((uint32_t)i, (uint32_t)s, pairs, b->num_tables[i], (int64_t)pairs <= b->wide_pairs, b->items,
-               b->npts[i]);
+      plan.add((uint32_t)i, (uint32_t)s, pairs, b->num_tables[i], (int64_t)pairs <= batch_wide_pairs(b, i),
+               b->items_i[i], b->npts[i]);
       if (plan.partial_tiles() > 0x7FFFFFFFull || plan.fold_tiles > 0x7FFFFFFFull)
         return fail(SC_ERR_UNSUPPORTED, "batch stage has too many tiles");
     }
@@ -1222,7 +1247,7 @@ int build_stages(sc_batch* b) {
     max_tiles = std::max(max_tiles, plan.partial_tiles());
     b->tail_offset[s] = (uint32_t)all.size();
     if (s == last) {
-      for (int i = 0; i < b->n; ++i) all.push_back(StageEntry{(uint32_t)i, (uint32_t)b->tail_round[i], 0u, 0u});
+      for (int i = 0; i < b->n; ++i) all.push_back(StageEntry{(uint32_t)i, (uint32_t)b->tail_round[i], 0u, 0u, 1u});
     }
     b->tail_entries[s] = (uint32_t)all.size() - b->tail_offset[s];
   }
@@ -1254,7 +1279,7 @@ int enqueue_batch(sc_batch* b) {
   for (int st = 0; st < b->num_stages; ++st) {
     BatchArgs a;
     a.insts = b->d_insts;
-    a.items = b->items;
+    a.pad1 = 0;
     a.partials = b->d_partials;
     a.fc = b->ctx->d_fc[b->fidx];
     a.tile_base = 0;
@@ -1329,13 +1354,9 @@ int sc_batch_create(sc_context* ctx, int32_t bit_width, int32_t n, const sc_comp
   }
   b->tmax = pick_tmax(maxT);
   b->pmax = maxP <= 4 ? 4 : 8;
-  b->wide_pairs = default_batch_wide_pairs(b->tmax);
-  for (int i = 0; i < n; ++i) {  // first round whose pairs fit the tail (as for a single proof)
-    const int64_t tp = default_tail_pairs(b->L, b->tmax > 8);
-    int j = 0;
-    while (j < b->num_vars[i] - 1 && (int64_t)(b->n0[i] >> (j + 1)) > tp) ++j;
-    b->tail_round[i] = j;
-  }
+  b->items_i.assign(n, 4u);
+  b->wide_i.assign(n, -1);
+  b->tail_i.assign(n, 0);
   SC_HIP_B(hipSetDevice(ctx->device));
   SC_HIP_B(hipMalloc(&b->pool, total));
   b->pool_bytes = total;
@@ -1385,9 +1406,7 @@ int sc_batch_create(sc_context* ctx, int32_t bit_width, int32_t n, const sc_comp
 
 int sc_batch_set_point_parallel(sc_batch* b, int64_t max_pairs) {
   if (b == nullptr) return fail(SC_ERR_ARG, "batch must not be NULL");
-  const int64_t w = max_pairs < 0 ? default_batch_wide_pairs(b->tmax) : max_pairs;
-  if (w == b->wide_pairs) return SC_OK;
-  b->wide_pairs = w;
+  b->wide_i.assign(b->n, max_pairs < 0 ? -1 : max_pairs);
   SC_HIP(hipStreamSynchronize(b->ctx->stream));
   return build_stages(b);
 }
@@ -1395,10 +1414,36 @@ int sc_batch_set_point_parallel(sc_batch* b, int64_t max_pairs) {
 int sc_batch_set_tile(sc_batch* b, int32_t items_per_thread) {
   if (b == nullptr) return fail(SC_ERR_ARG, "batch must not be NULL");
   if (items_per_thread < 1 || items_per_thread > 64) return fail(SC_ERR_ARG, "items_per_thread must be in [1, 64]");
-  if ((uint32_t)items_per_thread == b->items) return SC_OK;
-  b->items = (uint32_t)items_per_thread;
+  b->items_i.assign(b->n, (uint32_t)items_per_thread);
   SC_HIP(hipStreamSynchronize(b->ctx->stream));
   return build_stages(b);
+}
+
+int sc_batch_set_schedules(sc_batch* b, const int32_t* items_per_thread, const int64_t* max_pairs,
+                           const int64_t* tail_pairs) {
+  if (b == nullptr) return fail(SC_ERR_ARG, "batch must not be NULL");
+  for (int i = 0; i < b->n; ++i) {
+    if (items_per_thread && (items_per_thread[i] < 0 || items_per_thread[i] > 64))
+      return fail(SC_ERR_ARG, "items_per_thread must be in [0, 64]");
+    if (tail_pairs && tail_pairs[i] < 0) return fail(SC_ERR_ARG, "tail_pairs must be >= 0");
+  }
+  for (int i = 0; i < b->n; ++i) {
+    if (items_per_thread && items_per_thread[i] > 0) b->items_i[i] = (uint32_t)items_per_thread[i];
+    if (max_pairs) b->wide_i[i] = max_pairs[i] < 0 ? -1 : max_pairs[i];
+    if (tail_pairs) b->tail_i[i] = tail_pairs[i];
+  }
+  SC_HIP(hipStreamSynchronize(b->ctx->stream));
+  return build_stages(b);
+}
+
+int sc_batch_instance_schedule(sc_batch* b, int32_t i, int32_t* items_per_thread, int64_t* max_pairs,
+                               int32_t* tail_round) {
+  if (b == nullptr) return fail(SC_ERR_ARG, "batch must not be NULL");
+  if (i < 0 || i >= b->n) return fail(SC_ERR_ARG, "instance index out of range");
+  if (items_per_thread) *items_per_thread = (int32_t)b->items_i[i];
+  if (max_pairs) *max_pairs = batch_wide_pairs(b, i);
+  if (tail_round) *tail_round = b->tail_round[i];
+  return SC_OK;
 }
 
 int sc_batch_run(sc_batch* b, int32_t mode) {
@@ -1554,6 +1599,94 @@ struct Arena {
 
 enum ReqState { REQ_QUEUED = 0, REQ_RUNNING = 1, REQ_DONE = 2, REQ_FINISHING = 3 };
 
+// A server's device memory: chunks allocated on demand up to the budget, sizes doubling from
+// 4 GiB (so the whole budget takes a handful of allocations, made only as the load requires;
+// one allocation of the whole 200 GB budget took 5.3 s at creation). First fit within a
+// chunk; a request never spans chunks, and one larger than the next chunk size gets a chunk of
+// its own, released when it finishes. Empty chunks are released when a new chunk would
+// exceed the budget.
+struct ChunkArena {
+  struct Chunk {
+    uint8_t* base = nullptr;
+    uint64_t size = 0;
+    bool dedicated = false;
+    Arena fit;
+  };
+  std::map<int, Chunk> chunks;
+  int next_id = 0;
+  uint64_t budget = 0, held = 0, next_size = uint64_t(4) << 30;
+
+  uint64_t used() const {
+    uint64_t u = 0;
+    for (const auto& kv : chunks) u += kv.second.fit.used;
+    return u;
+  }
+  // SC_OK with *chunk, *off and *ptr set; 1 when it does not fit now; SC_ERR_* on failure.
+  int alloc(uint64_t n, int* chunk, uint64_t* off, uint8_t** ptr) {
+    n = (n + 255) & ~(uint64_t)255;
+    for (auto& kv : chunks) {
+      if (kv.second.dedicated) continue;
+      if (kv.second.fit.alloc(n, *off)) {
+        *chunk = kv.first;
+        *ptr = kv.second.base + *off;
+        return SC_OK;
+      }
+    }
+    const bool dedicated = n > next_size;
+    uint64_t size = dedicated ? ((n + (uint64_t(256) << 20) - 1) & ~((uint64_t(256) << 20) - 1)) : next_size;
+    if (held + size > budget) {
+      release_empty();
+      if (held + size > budget) {
+        if (!dedicated && held + n <= budget) {
+          size = (budget - held) & ~(uint64_t)255;  // the budget's remainder as a last chunk
+        } else {
+          return 1;
+        }
+      }
+    }
+    Chunk c;
+    const hipError_t e = hipMalloc(&c.base, size);
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      return 1;
+    }
+    if (e != hipSuccess) return fail(SC_ERR_HIP, std::string("server arena chunk: ") + hipGetErrorString(e));
+    c.size = size;
+    c.dedicated = dedicated;
+    c.fit.init(c.base, size);
+    held += size;
+    if (!dedicated && next_size < (uint64_t(64) << 30)) next_size *= 2;
+    const int id = next_id++;
+    auto& slot = chunks[id] = c;
+    if (!slot.fit.alloc(n, *off)) return fail(SC_ERR_HIP, "server arena: new chunk too small");
+    *chunk = id;
+    *ptr = slot.base + *off;
+    return SC_OK;
+  }
+  void release(int chunk, uint64_t off, uint64_t n) {
+    auto it = chunks.find(chunk);
+    if (it == chunks.end()) return;
+    it->second.fit.release(off, n);
+    if (it->second.dedicated && it->second.fit.used == 0) drop(it);
+  }
+  void release_empty() {
+    for (auto it = chunks.begin(); it != chunks.end();) {
+      auto cur = it++;
+      if (cur->second.fit.used == 0) drop(cur);
+    }
+  }
+  void drop(std::map<int, Chunk>::iterator it) {
+    (void)hipFree(it->second.base);
+    held -= it->second.size;
+    chunks.erase(it);
+  }
+  void free_all() {
+    for (auto& kv : chunks) (void)hipFree(kv.second.base);
+    chunks.clear();
+    held = 0;
+  }
+};
+
 struct Request {
   int64_t id = 0;
   int fidx = 0, L = 0, npts = 0, num_vars = 0, mode = SC_CHALLENGE_SHA3;
@@ -1562,7 +1695,10 @@ struct Request {
   int state = REQ_QUEUED;
   int slot = -1;
   int round = 0;  // next round to run
+  uint32_t items = 4;  // tile of its stage entries (pairs per thread)
+  int64_t wide = -1;   // rounds with at most this many pairs point-parallel; -1: default
   uint64_t arena_off = 0, arena_bytes = 0;
+  int arena_chunk = -1;
   ProofState* result = nullptr;  // pinned host copy of the final state
 };
 
@@ -1578,8 +1714,7 @@ struct sc_server {
   sc_context* ctx = nullptr;
   int max_active = 0;
   uint32_t items = 4;
-  uint8_t* arena_mem = nullptr;
-  Arena arena;
+  ChunkArena arena;
   InstDesc* d_insts = nullptr;       // [max_active]
   CompDesc* d_comps = nullptr;       // [max_active]
   ProofState* d_states = nullptr;    // [max_active]
@@ -1619,7 +1754,7 @@ void free_server(sc_server* sv) {
   }
   for (auto& r : sv->inflight)
     if (r.ev) (void)hipEventDestroy(r.ev);
-  if (sv->arena_mem) (void)hipFree(sv->arena_mem);
+  sv->arena.free_all();
   if (sv->d_insts) (void)hipFree(sv->d_insts);
   if (sv->d_comps) (void)hipFree(sv->d_comps);
   if (sv->d_states) (void)hipFree(sv->d_states);
@@ -1667,7 +1802,7 @@ int retire(sc_server* sv, bool wait) {
     for (int64_t id : rec.finishing) {
       Request& r = sv->reqs[id];
       r.state = REQ_DONE;
-      sv->arena.release(r.arena_off, r.arena_bytes);
+      sv->arena.release(r.arena_chunk, r.arena_off, r.arena_bytes);
       sv->free_slots.push_back(r.slot);
       sv->slot_req[r.slot] = 0;
       r.slot = -1;
@@ -1678,9 +1813,35 @@ int retire(sc_server* sv, bool wait) {
   return SC_OK;
 }
 
+// A request's largest round (round 0, point-parallel: the most tiles) must fit the step's
+// partials buffer on its own, or it could never run: raise its tile until it does, or refuse.
+int fit_request_tiles(const sc_server* sv, Request& r) {
+  const uint64_t pairs0 = ((uint64_t)1 << r.num_vars) >> 1;
+  while (StagePlan::tiles(pairs0, true, r.items, r.npts, 0) > sv->max_tiles) {
+    if (r.items >= 64) return fail(SC_ERR_UNSUPPORTED, "request too large for one server step (num_vars)");
+    r.items *= 2;
+  }
+  return SC_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int sc_server_set_request_schedule(sc_server* sv, int64_t handle, int32_t items_per_thread, int64_t max_pairs) {
+  if (sv == nullptr) return fail(SC_ERR_ARG, "server must not be NULL");
+  auto it = sv->reqs.find(handle);
+  if (it == sv->reqs.end() || it->second.state == REQ_DONE) return fail(SC_ERR_ARG, "no such queued or running request");
+  if (items_per_thread < 0 || items_per_thread > 64) return fail(SC_ERR_ARG, "items_per_thread must be in [0, 64]");
+  Request r = it->second;
+  if (items_per_thread > 0) r.items = (uint32_t)items_per_thread;
+  r.wide = max_pairs < 0 ? -1 : max_pairs;
+  const int rc = fit_request_tiles(sv, r);
+  if (rc != SC_OK) return rc;
+  it->second.items = r.items;
+  it->second.wide = r.wide;
+  return SC_OK;
+}
 
 int sc_server_create(sc_context* ctx, uint64_t hbm_budget_bytes, int32_t max_active, sc_server** out) {
   if (ctx == nullptr || out == nullptr) return fail(SC_ERR_ARG, "NULL argument");
@@ -1702,8 +1863,7 @@ int sc_server_create(sc_context* ctx, uint64_t hbm_budget_bytes, int32_t max_act
                   std::string(#expr) + ": " + hipGetErrorString(_e));                         \
     }                                                                                         \
   } while (0)
-  SC_HIP_S(hipMalloc(&sv->arena_mem, hbm_budget_bytes));
-  sv->arena.init(sv->arena_mem, hbm_budget_bytes);
+  sv->arena.budget = hbm_budget_bytes;
   SC_HIP_S(hipMalloc(&sv->d_insts, sizeof(InstDesc) * max_active));
   SC_HIP_S(hipMalloc(&sv->d_comps, sizeof(CompDesc) * max_active));
   SC_HIP_S(hipMalloc(&sv->d_states, sizeof(ProofState) * max_active));
@@ -1744,7 +1904,10 @@ int sc_server_submit(sc_server* sv, int32_t bit_width, const sc_composition* com
   r.num_vars = num_vars;
   r.mode = challenge_mode;
   r.input = *input;
-  if (request_bytes(r) > sv->arena.size) return fail(SC_ERR_MEMORY, "request exceeds the HBM budget");
+  if (request_bytes(r) > sv->arena.budget) return fail(SC_ERR_MEMORY, "request exceeds the HBM budget");
+  r.items = sv->items;
+  rc = fit_request_tiles(sv, r);
+  if (rc != SC_OK) return rc;
   SC_HIP(hipHostMalloc(&r.result, sizeof(ProofState), hipHostMallocDefault));
   sv->reqs[r.id] = r;
   sv->queue.push_back(r.id);
@@ -1771,20 +1934,31 @@ int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
     Request& r = sv->reqs[sv->queue.front()];
     const uint64_t bytes = request_bytes(r);
     uint64_t off = 0;
-    if (!sv->arena.alloc(bytes, off)) break;
+    int chunk = -1;
+    uint8_t* base = nullptr;
+    const int arc = sv->arena.alloc(bytes, &chunk, &off, &base);
+    if (arc != SC_OK && arc != 1) return arc;
+    if (arc == 1) {
+      // does not fit now; with nothing running it never will (the device holds other memory)
+      bool running = false;
+      for (int64_t id : sv->slot_req) running = running || id != 0;
+      if (!running && sv->inflight.empty())
+        return fail(SC_ERR_MEMORY, "server arena: the device cannot hold the next request");
+      break;
+    }
     sv->queue.pop_front();
     r.slot = sv->free_slots.back();
     sv->free_slots.pop_back();
     sv->slot_req[r.slot] = r.id;
     r.arena_off = off;
     r.arena_bytes = bytes;
+    r.arena_chunk = chunk;
     r.state = REQ_RUNNING;
     r.round = 0;
     const Launchers& ln = launchers_for(r.fidx);
     const uint64_t n0 = (uint64_t)1 << r.num_vars;
     const uint64_t words = (uint64_t)r.desc.num_tables * n0 * r.L;
     auto align = [](uint64_t x) { return (x + 255) & ~(uint64_t)255; };
-    uint8_t* base = sv->arena.base + off;
     const int si = take_staging(sv);
     if (si < 0) return fail(SC_ERR_HIP, "no staging buffer");
     auto& st = sv->staging[si];
@@ -1833,7 +2007,9 @@ int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
     }
     if (slots.empty()) continue;
     const int tmax = pick_tmax(maxT), pmax = maxP <= 4 ? 4 : 8;
-    const int64_t wide_pairs = default_batch_wide_pairs(tmax);
+    // oldest request first: one that did not fit a step's partials gets them in the next
+    std::sort(slots.begin(), slots.end(),
+              [&](int a2, int b2) { return sv->slot_req[a2] < sv->slot_req[b2]; });
     // one round of each request (per-pair or point-parallel), then tail entries: requests
     // whose current round has at most the tail's pairs finish all their remaining rounds
     StagePlan plan;
@@ -1843,13 +2019,14 @@ int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
       const uint64_t pairs = ((uint64_t)1 << r.num_vars) >> (r.round + 1);
       const int64_t tail_pairs = default_tail_pairs(r.L, pick_tmax(r.desc.num_tables) > 8);
       if ((int64_t)pairs <= tail_pairs) {
-        tails.push_back(StageEntry{(uint32_t)slot, (uint32_t)r.round, 0u, 0u});
+        tails.push_back(StageEntry{(uint32_t)slot, (uint32_t)r.round, 0u, 0u, 1u});
         continue;
       }
+      const int64_t wide_pairs = r.wide < 0 ? default_instance_wide_pairs(r.L, tmax, r.npts) : r.wide;
       const bool wide = (int64_t)pairs <= wide_pairs;
-      if (plan.partial_tiles() + StagePlan::tiles(pairs, wide, sv->items, r.npts, (uint32_t)r.round) > sv->max_tiles)
-        continue;  // runs in a later step
-      plan.add((uint32_t)slot, (uint32_t)r.round, pairs, r.desc.num_tables, wide, sv->items, r.npts);
+      if (plan.partial_tiles() + StagePlan::tiles(pairs, wide, r.items, r.npts, (uint32_t)r.round) > sv->max_tiles)
+        continue;  // runs in a later step (alone it fits: fit_request_tiles)
+      plan.add((uint32_t)slot, (uint32_t)r.round, pairs, r.desc.num_tables, wide, r.items, r.npts);
     }
     if (plan.size() == 0 && tails.empty()) continue;
     std::vector<StageEntry> ents;
@@ -1869,7 +2046,7 @@ int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
     a.tile_base = 0;
     a.pad0 = 0;
     a.insts = sv->d_insts;
-    a.items = sv->items;
+    a.pad1 = 0;
     a.partials = sv->d_partials;
     a.fc = ctx->d_fc[fidx];
     const Launchers& ln = launchers_for(fidx);
@@ -1954,8 +2131,8 @@ int sc_server_result(sc_server* sv, int64_t handle, uint32_t* round_polys, uint3
 
 int sc_server_info(sc_server* sv, uint64_t* arena_used, uint64_t* arena_size, uint64_t* steps) {
   if (sv == nullptr) return fail(SC_ERR_ARG, "server must not be NULL");
-  if (arena_used) *arena_used = sv->arena.used;
-  if (arena_size) *arena_size = sv->arena.size;
+  if (arena_used) *arena_used = sv->arena.used();
+  if (arena_size) *arena_size = sv->arena.held;
   if (steps) *steps = sv->steps;
   return SC_OK;
 }

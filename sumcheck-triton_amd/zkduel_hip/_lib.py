@@ -97,6 +97,10 @@ SIGNATURES = [
       ctypes.POINTER(ScInput), ctypes.POINTER(ctypes.c_void_p)]),
     ("sc_batch_set_tile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("sc_batch_set_point_parallel", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    ("sc_batch_set_schedules", ctypes.c_int,
+     [ctypes.c_void_p, i32p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+    ("sc_batch_instance_schedule", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int32, i32p, ctypes.POINTER(ctypes.c_int64), i32p]),
     ("sc_batch_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("sc_batch_capture", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("sc_batch_wait", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
@@ -109,6 +113,7 @@ SIGNATURES = [
       ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
     ("sc_server_step", ctypes.c_int, [ctypes.c_void_p, i32p, i32p]),
     ("sc_server_poll", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, i32p]),
+    ("sc_server_set_request_schedule", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64]),
     ("sc_server_drain", ctypes.c_int, [ctypes.c_void_p]),
     ("sc_server_result", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, u32p, u32p, u64p]),
     ("sc_server_info", ctypes.c_int, [ctypes.c_void_p, u64p, u64p, u64p]),

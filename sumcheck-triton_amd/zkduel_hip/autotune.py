@@ -1,5 +1,5 @@
-"""Schedule autotuner: picks a proof's launch schedule, and a batch's tile size, by timing the
-candidates on the device for the actual (width, composition, size).
+"""Schedule autotuner: picks a proof's launch schedule, and a batch's per-class schedules, by
+timing the candidates on the device for the actual (width, composition, size).
 
 Knobs, none of which changes any result bit:
 
@@ -8,7 +8,9 @@ Knobs, none of which changes any result bit:
   kernel (the "fusion depth": how many final rounds run in one launch);
 * ``wide_pairs`` — rounds with at most this many pairs run as a fold launch plus the
   point-parallel evaluation instead of the per-pair pass kernel (-1: the library's default);
-* ``items`` (batches) — pairs per thread in a batch tile.
+* ``items`` (batches and served requests) — pairs per thread in a stage tile, per (width,
+  degree, size) class of instance (``tune_batch``; a Server applies a tuned class table to
+  the requests it admits).
 
 The library's default schedule (0 / 0 / -1) is always a candidate and is kept unless the
 best tuned schedule beats it by more than ``margin`` (2% by default): candidates closer than
@@ -131,7 +133,91 @@ def tune_proof(proof: Proof, challenge_mode: str = "sha3", repeats: int = 3,
     return choice
 
 
-def tune_batch(batch: Batch, challenge_mode: str = "sha3", items: Iterable[int] = (1, 2, 4, 8, 16, 32),
-               repeats: int = 1) -> dict[int, float]:
-    """Tile size of a batch (pairs per thread), chosen by timing the actual mix."""
-    return batch.autotune(tuple(items), challenge_mode, repeats)
+def instance_class(bit_width: int, comp, num_vars: int) -> str:
+    """The (width, degree, size) class a batched or served instance is tuned by: degree 2, 3,
+    4-5 or 6-7; num_vars up to 14, 15-18 or 19 and more; more than 8 tables or not."""
+    d = comp.degree
+    dc = "d2" if d <= 2 else ("d3" if d == 3 else ("d4-5" if d <= 5 else "d6-7"))
+    sc = "n<=14" if num_vars <= 14 else ("n15-18" if num_vars <= 18 else "n>=19")
+    tc = "t>8" if comp.num_tables > 8 else "t<=8"
+    return f"{bit_width}/{dc}/{sc}/{tc}"
+
+
+def _batch_ms(batch: Batch, mode: str, repeats: int) -> float:
+    batch.run(mode)
+    batch.wait()
+    best = float("inf")
+    for _ in range(max(1, repeats)):
+        batch.run(mode)
+        best = min(best, batch.wait())
+    return best
+
+
+BATCH_ITEMS = (1, 2, 4, 8, 16)
+BATCH_WIDE = (-1, 0, 1 << 40)
+BATCH_TAIL = (0, 16, 4096)
+
+
+def tune_batch(batch: Batch, challenge_mode: str = "sha3", items: Iterable[int] = BATCH_ITEMS,
+               wide_pairs: Iterable[int] = BATCH_WIDE, tail_pairs: Iterable[int] = BATCH_TAIL,
+               repeats: int = 2, path: str | os.PathLike | None = None, margin: float = 0.01) -> dict:
+    """Per-class schedules of a batch: for each (width, degree, size) class of its instances
+    (instance_class), the tile (pairs per thread), the kernel choice (point-parallel threshold:
+    -1 the library's rule, 0 per-pair, 2^40 every round point-parallel) and the fusion depth of
+    its tail (pairs at or below which its last rounds run in the batch's one tail launch: 0 the
+    default). Classes are tuned one at a time, largest work first, each candidate timed on the
+    whole batch with the other classes at their current choice (the stage launches are
+    shared, so a class's choice is judged by the batch's time); a candidate replaces the
+    current choice when it is faster by more than ``margin``. Sets the choices on the batch
+    and returns {"classes": {class: {"items", "max_pairs", "tail_pairs", "instances"}}, "ms",
+    "default_ms", "candidates"}. Classes cached (per process, and in ``path`` when given) are
+    applied first and not re-tuned."""
+    p = Path(path) if path is not None else None
+    _load(p)
+    bw = batch.field.bit_width
+    n = len(batch)
+    cls_of = [instance_class(bw, batch.comps[i], batch.num_vars[i]) for i in range(n)]
+    members: dict[str, list[int]] = {}
+    for i, c in enumerate(cls_of):
+        members.setdefault(c, []).append(i)
+    work = {c: sum((batch.comps[i].num_tables * (1 << batch.num_vars[i])) for i in idx) for c, idx in members.items()}
+    choice = {c: {"items": 4, "max_pairs": -1, "tail_pairs": 0} for c in members}
+    for c in members:
+        if f"batch/{c}" in _cache:
+            choice[c] = {k: _cache[f"batch/{c}"][k] for k in ("items", "max_pairs", "tail_pairs")}
+
+    def apply() -> None:
+        batch.set_schedules([choice[c]["items"] for c in cls_of], [choice[c]["max_pairs"] for c in cls_of],
+                            [choice[c]["tail_pairs"] for c in cls_of])
+
+    apply()
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < 0.1:  # warm the device up before the first timing
+        batch.run(challenge_mode)
+        batch.wait()
+    default_ms = _batch_ms(batch, challenge_mode, repeats)
+    best_ms = default_ms
+    candidates: dict[str, float] = {}
+    for c in sorted(members, key=lambda c: -work[c]):
+        if f"batch/{c}" in _cache:
+            continue
+        for knob, values in (("max_pairs", tuple(wide_pairs)), ("items", tuple(items)),
+                             ("tail_pairs", tuple(tail_pairs))):
+            current = choice[c][knob]
+            for v in values:
+                if v == current:
+                    continue
+                choice[c][knob] = v
+                apply()
+                ms = _batch_ms(batch, challenge_mode, repeats)
+                candidates[f"{c}:{knob}={v}"] = round(ms, 3)
+                if ms < best_ms * (1.0 - margin):
+                    best_ms, current = ms, v
+                choice[c][knob] = current
+            apply()
+        _cache[f"batch/{c}"] = dict(choice[c])
+    _store(p)
+    apply()
+    final_ms = _batch_ms(batch, challenge_mode, repeats)
+    return {"classes": {c: dict(choice[c], instances=len(members[c])) for c in sorted(members)},
+            "ms": final_ms, "default_ms": default_ms, "candidates": candidates}

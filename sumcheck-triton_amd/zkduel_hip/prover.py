@@ -392,6 +392,34 @@ class Batch:
         negative: the default rule)."""
         _lib.check(self.lib.sc_batch_set_point_parallel(self.handle, int(max_pairs)))
 
+    def set_schedules(self, items: Sequence[int] | None = None, max_pairs: Sequence[int] | None = None,
+                      tail_pairs: Sequence[int] | None = None) -> None:
+        """Per-instance schedules, one value per instance (None keeps the current ones): tile
+        (pairs per thread; 0 keeps), point-parallel threshold (negative: default) and tail
+        threshold in pairs (0: default). Results do not depend on them."""
+        n = len(self.comps)
+
+        def arr(vals, dtype):
+            if vals is None:
+                return None
+            a = np.ascontiguousarray(vals, dtype=dtype)
+            if a.shape != (n,):
+                raise ValueError(f"need {n} values, one per instance")
+            return a
+
+        it, mp, tp = arr(items, np.int32), arr(max_pairs, np.int64), arr(tail_pairs, np.int64)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        _lib.check(self.lib.sc_batch_set_schedules(
+            self.handle, it.ctypes.data_as(_lib.i32p) if it is not None else None,
+            mp.ctypes.data_as(i64p) if mp is not None else None, tp.ctypes.data_as(i64p) if tp is not None else None))
+
+    def instance_schedule(self, i: int) -> dict:
+        """{"items", "point_parallel_pairs", "tail_round"} of instance i, as in effect."""
+        it, mp, tr = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32()
+        _lib.check(self.lib.sc_batch_instance_schedule(self.handle, int(i), ctypes.byref(it), ctypes.byref(mp),
+                                                       ctypes.byref(tr)))
+        return {"items": it.value, "point_parallel_pairs": mp.value, "tail_round": tr.value}
+
     def run(self, challenge_mode: str = "sha3") -> None:
         _lib.check(self.lib.sc_batch_run(self.handle, _lib.CHALLENGE_MODES[challenge_mode]))
 

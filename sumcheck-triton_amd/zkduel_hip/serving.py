@@ -24,9 +24,15 @@ STATES = {0: "queued", 1: "running", 2: "done"}
 
 
 class Server:
-    def __init__(self, ctx: Context, hbm_budget_bytes: int, max_active: int = 256):
+    """``schedules``: optional per-class schedules, {class key: {"items", "max_pairs"}} with keys
+    from autotune.instance_class (as autotune.tune_batch returns them under "classes"); a
+    submitted request of a listed class gets that schedule."""
+
+    def __init__(self, ctx: Context, hbm_budget_bytes: int, max_active: int = 256,
+                 schedules: dict | None = None):
         self.ctx = ctx
         self.lib = ctx.lib
+        self.schedules = dict(schedules or {})
         handle = ctypes.c_void_p()
         _lib.check(self.lib.sc_server_create(ctx.handle, int(hbm_budget_bytes), int(max_active), ctypes.byref(handle)))
         self.handle = handle
@@ -55,7 +61,18 @@ class Server:
                                              _lib.CHALLENGE_MODES[challenge_mode], ctypes.byref(h)))
         del keep
         self._shape[h.value] = (bit_width, int(num_vars), comp.degree + 1)
+        if self.schedules:
+            from .autotune import instance_class
+
+            sched = self.schedules.get(instance_class(bit_width, comp, int(num_vars)))
+            if sched is not None:
+                self.set_request_schedule(h.value, sched.get("items", 0), sched.get("max_pairs", -1))
         return h.value
+
+    def set_request_schedule(self, handle: int, items: int = 0, max_pairs: int = -1) -> None:
+        """Tile (pairs per thread; 0 keeps) and point-parallel threshold (negative: default) of a
+        queued or running request. Results do not depend on them."""
+        _lib.check(self.lib.sc_server_set_request_schedule(self.handle, int(handle), int(items), int(max_pairs)))
 
     def step(self) -> tuple[int, int]:
         running, queued = ctypes.c_int32(), ctypes.c_int32()
@@ -87,7 +104,7 @@ class Server:
 
 
 def run_poisson(ctx: Context, requests: Sequence[dict], hbm_budget_bytes: int, max_active: int = 256,
-                challenge_mode: str = "sha3") -> dict:
+                challenge_mode: str = "sha3", schedules: dict | None = None) -> dict:
     """Replay ``requests`` (dicts with arrival, bit_width, num_vars, composition, seed,
     instance; see synthetic.serving_requests) against a Server in real time. Returns
     throughput, latency percentiles and every request's checksum."""
@@ -95,7 +112,7 @@ def run_poisson(ctx: Context, requests: Sequence[dict], hbm_budget_bytes: int, m
     checksums: dict[int, int] = {}
     latency: dict[int, float] = {}
     t_setup = time.perf_counter()
-    sv = Server(ctx, hbm_budget_bytes, max_active)
+    sv = Server(ctx, hbm_budget_bytes, max_active, schedules)
     setup_s = time.perf_counter() - t_setup
     try:
         live: dict[int, dict] = {}

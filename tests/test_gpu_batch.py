@@ -104,3 +104,39 @@ def test_batch_graph_replays(ctx):
             for i in range(len(insts)):
                 got = b.result(i)
                 assert np.array_equal(got[0], base[i][0]) and got[2] == base[i][2]
+
+
+@pytest.mark.parametrize("bw", [64, 256])
+def test_per_instance_schedules_keep_results(ctx, bw):
+    """Random per-instance tiles, kernel choices and tail depths (sc_batch_set_schedules), and
+    the per-class tuner, change no result bit; the schedule in effect reads back."""
+    from zkduel_hip.autotune import tune_batch
+
+    insts = batch_instances(n=32, seed=70 + bw, nv_lo=2, nv_hi=14)
+    wants = [plain_result(ctx, bw, comp, nv, inp, "sha3") for comp, nv, inp in insts]
+    rng = random.Random(bw)
+    with Batch(ctx, bw, insts) as b:
+        for trial in range(3):
+            items = [rng.choice((1, 2, 4, 8, 16)) for _ in insts]
+            wide = [rng.choice((-1, 0, 1 << 6, 1 << 40)) for _ in insts]
+            tails = [rng.choice((0, 1, 16, 4096)) for _ in insts]
+            b.set_schedules(items, wide, tails)
+            for i in range(len(insts)):
+                sched = b.instance_schedule(i)
+                assert sched["items"] == items[i]
+                if wide[i] >= 0:
+                    assert sched["point_parallel_pairs"] == wide[i]
+            say("per-instance schedules", bw, trial)
+            b.run("sha3")
+            b.wait()
+            for i in range(len(insts)):
+                got = b.result(i)
+                assert np.array_equal(got[0], wants[i][0]) and np.array_equal(got[1], wants[i][1]), (trial, i)
+        tuned = tune_batch(b, items=(2, 8), wide_pairs=(-1, 0), tail_pairs=(0, 16), repeats=1)
+        say("tuned classes", tuned["classes"])
+        assert sum(c["instances"] for c in tuned["classes"].values()) == len(insts)
+        b.run("sha3")
+        b.wait()
+        for i in range(len(insts)):
+            got = b.result(i)
+            assert np.array_equal(got[0], wants[i][0]) and got[2] == wants[i][2], i

@@ -101,3 +101,67 @@ def test_poisson_replay_results_match_alone(ctx):
     for r in reqs:
         want = alone(ctx, r["bit_width"], r["composition"], r["num_vars"], r["seed"], r["instance"])[2]
         assert stats["checksums"][r["id"]] == want
+
+
+def test_request_schedules_keep_results(ctx):
+    reqs = mixed_requests(16, 4)
+    rng = random.Random(4)
+    with Server(ctx, 1 << 30, max_active=8) as sv:
+        handles = {}
+        for bw, comp, nv, i in reqs:
+            h = sv.submit(bw, comp, nv, {"seed": 13, "instance": i})
+            sv.set_request_schedule(h, rng.choice((1, 2, 8, 32)), rng.choice((-1, 0, 1 << 40)))
+            handles[h] = (bw, comp, nv, i)
+        while any(sv.poll(h) != "done" for h in handles):
+            sv.step()
+        for h, (bw, comp, nv, i) in handles.items():
+            got = sv.result(h)
+            want = alone(ctx, bw, comp, nv, 13, i)
+            assert np.array_equal(got[0], want[0]) and got[2] == want[2], (bw, nv)
+
+
+def test_request_too_large_for_one_step_is_refused_at_submit(ctx):
+    """A request whose largest round needs more partial tiles than a step holds even at the
+    largest tile is refused (it could never run); one that fits with a larger tile is
+    accepted. Nothing is stepped: no table memory is allocated."""
+    comp = Composition.from_terms(1, [(1, [0] * 7)])  # degree 7: 32 pairs per block
+    with Server(ctx, 200 * 10**9, max_active=4) as sv:
+        with pytest.raises(ValueError):
+            sv.submit(32, comp, 33, {"seed": 1, "instance": 0})  # 2^32 pairs in round 0
+        h = sv.submit(32, comp, 31, {"seed": 1, "instance": 1})  # fits with a tile of 32
+        assert sv.poll(h) == "queued"
+        assert sv.info()["arena_size"] == 0  # creation and submission hold no table memory
+
+
+def test_long_mixed_replay_returns_all_memory(ctx):
+    """400 requests of every width at a high arrival rate under a small budget: every result
+    equals the proof alone, and the arena holds nothing once all are done."""
+    rng = random.Random(12)
+    reqs = serving_requests(400, rate_per_s=20000.0, seed=12, nv_lo=6, nv_hi=16)
+    for r in reqs:
+        r["bit_width"] = rng.choice((32, 64, 128, 256))
+    budget = 1 << 28
+    with Server(ctx, budget, max_active=64) as sv:
+        live = {}
+        nxt = 0
+        peak = 0
+        while nxt < len(reqs) or live:
+            for _ in range(8):
+                if nxt < len(reqs):
+                    r = reqs[nxt]
+                    live[sv.submit(r["bit_width"], r["composition"], r["num_vars"],
+                                   {"seed": r["seed"], "instance": r["instance"]})] = r
+                    nxt += 1
+            sv.step()
+            info = sv.info()
+            peak = max(peak, info["arena_size"])
+            assert info["arena_used"] <= budget and info["arena_size"] <= budget
+            for h in [h for h in live if sv.poll(h) == "done"]:
+                r = live.pop(h)
+                r["got"] = sv.result(h)[2]
+        sv.drain()
+        info = sv.info()
+        say("replay", info, "peak held", peak)
+        assert info["arena_used"] == 0
+    for r in reqs:
+        assert r["got"] == alone(ctx, r["bit_width"], r["composition"], r["num_vars"], r["seed"], r["instance"])[2], r["id"]
