@@ -283,50 +283,59 @@ def _host_info() -> dict:
     return {"cpu_model": model, "nproc": os.cpu_count(), "python": sys.version.split()[0]}
 
 
-def cpu_baseline(budget_s: float = 20.0) -> dict:
-    """Oracle (single-core Python restatement of reference.py) on the same workload at the
-    largest num_vars that fits the budget; proofs/s scaled to num_vars 26 by 2^(nv - 26)
-    (the prover's work is linear in the table size). Also timed, on one core each: every
-    width of a*b*c + d*e at num_vars 16 (scaled to its benchmark size), the Goldilocks a*b*c
-    proof at num_vars 16 in full, and the BLS12-381 constraint workload at num_vars 12
-    (scaled to 22)."""
+def _pin_one_core() -> tuple[set, int]:
+    """Pins the calling thread to one CPU (the first it may run on), as `taskset -c` would;
+    returns the previous affinity set and the CPU."""
+    prev = os.sched_getaffinity(0)
+    cpu = min(prev)
+    os.sched_setaffinity(0, {cpu})
+    return prev, cpu
+
+
+def cpu_baseline() -> dict:
+    """The oracle (single-core Python restatement of reference.py:44-110) on the benchmark
+    workloads, pinned to one CPU, at the sizes of the baseline plan: a*b*c + d*e at num_vars 20
+    at every width (the headline value: Goldilocks, scaled to num_vars 26 by 2^(20 - 26), the
+    prover's work being linear in the table size), the Goldilocks a*b*c proof at num_vars 16 in
+    full, and the BLS12-381 constraint workload at num_vars 16 (scaled to 22). Table generation
+    is not timed; the SHA3 transcript is."""
     from zkduel_hip import get_spec
     from zkduel_hip.synthetic import constraint_composition
 
-    spec = get_spec(WORKLOAD)
-    terms = [(c, list(spec.term_vars[o : o + n])) for c, n, o in zip(spec.coeffs, spec.term_lens, spec.term_offsets)]
-    nv, secs = 8, 0.0
-    while True:
-        secs = _oracle_time(64, terms, spec.num_vars, nv)
-        if secs * 2.2 > budget_s or nv >= 20:
-            break
-        nv += 1
+    prev, cpu = _pin_one_core()
+    try:
+        spec = get_spec(WORKLOAD)
+        terms = [(c, list(spec.term_vars[o : o + n])) for c, n, o in zip(spec.coeffs, spec.term_lens, spec.term_offsets)]
+        nv = 20
+        per = {}
+        for bw in (64, 32, 128, 256):
+            t = _oracle_time(bw, terms, spec.num_vars, nv)
+            per[f"abc_plus_de_{bw}bit"] = {"num_vars_timed": nv, "seconds": round(t, 3),
+                                           "proofs_per_s_at_benchmark_size": (1.0 / t) * 2.0 ** (nv - SIZES[bw]),
+                                           "benchmark_num_vars": SIZES[bw]}
+        abc = get_spec("poly_abc")
+        abc_terms = [(c, list(abc.term_vars[o : o + n])) for c, n, o in zip(abc.coeffs, abc.term_lens, abc.term_offsets)]
+        t = _oracle_time(64, abc_terms, abc.num_vars, 16)
+        per["goldilocks_abc_nv16"] = {"num_vars_timed": 16, "seconds": round(t, 3), "proofs_per_s": 1.0 / t}
+        comp3, mask3 = constraint_composition(256)
+        t = _oracle_time(256, list(comp3.terms()), comp3.num_tables, 16, mask3)
+        per["bls_constraint_12_monomials"] = {"num_vars_timed": 16, "seconds": round(t, 3),
+                                              "proofs_per_s_at_benchmark_size": (1.0 / t) * 2.0 ** (16 - 22),
+                                              "benchmark_num_vars": 22}
+    finally:
+        os.sched_setaffinity(0, prev)
+    secs = per["abc_plus_de_64bit"]["seconds"]
     full = SIZES[64]
-    per = {}
-    for bw in (32, 64, 128, 256):
-        t = _oracle_time(bw, terms, spec.num_vars, 16)
-        per[f"abc_plus_de_{bw}bit"] = {"num_vars_timed": 16, "seconds": round(t, 3),
-                                       "proofs_per_s_at_benchmark_size": (1.0 / t) * 2.0 ** (16 - SIZES[bw]),
-                                       "benchmark_num_vars": SIZES[bw]}
-    abc = get_spec("poly_abc")
-    abc_terms = [(c, list(abc.term_vars[o : o + n])) for c, n, o in zip(abc.coeffs, abc.term_lens, abc.term_offsets)]
-    t = _oracle_time(64, abc_terms, abc.num_vars, 16)
-    per["goldilocks_abc_nv16"] = {"num_vars_timed": 16, "seconds": round(t, 3), "proofs_per_s": 1.0 / t}
-    comp3, mask3 = constraint_composition(256)
-    t = _oracle_time(256, list(comp3.terms()), comp3.num_tables, 12, mask3)
-    per["bls_constraint_12_monomials"] = {"num_vars_timed": 12, "seconds": round(t, 3),
-                                          "proofs_per_s_at_benchmark_size": (1.0 / t) * 2.0 ** (12 - 22),
-                                          "benchmark_num_vars": 22}
     return {
         "value": (1.0 / secs) * 2.0 ** (nv - full),
         "unit": "proofs/s",
         "cores": 1,
         "kind": "port",
         "sample": f"oracle/sumcheck_ref.py prove(), Goldilocks a*b*c+d*e, 5 uniform tables, SHA3, "
-                  f"num_vars={nv} took {secs:.2f} s on 1 core; scaled by 2^({nv}-{full})",
+                  f"num_vars={nv} took {secs:.2f} s on 1 core (pinned to CPU {cpu}); scaled by 2^({nv}-{full})",
         "per_workload": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                          for k, v in per.items()},
-        "host": _host_info(),
+        "host": dict(_host_info(), pinned_cpu=cpu),
     }
 
 

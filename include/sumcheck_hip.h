@@ -140,6 +140,11 @@ int sc_proof_use_compiled(sc_proof* proof, int32_t enable);
  * returns, for the last run, pass_ms[j] for rounds j < pass_rounds and *tail_ms. */
 int sc_proof_set_profiling(sc_proof* proof, int32_t enable);
 int sc_proof_kernel_times(sc_proof* proof, float* pass_ms, int32_t capacity, float* tail_ms);
+/* As sc_proof_kernel_times, plus finalize_ms[j]: round j's finalize launch (block partials
+ * summed, g_j stored, r_j derived on the device), the reference's per-round challenge phase
+ * (field_sumcheck.cu:1053-1066) in round-sum timing. Any pointer may be NULL. */
+int sc_proof_round_times(sc_proof* proof, float* pass_ms, float* finalize_ms, int32_t capacity,
+                         float* tail_ms);
 
 /* Final-point step of the last run (no counterpart in the reference, which stops after the
  * last round polynomial, field_sumcheck.cu:1057): r_final [L] = the challenge derived from the last

@@ -278,6 +278,7 @@ struct sc_proof {
   bool profiling = false;
   bool profiled_last = false;
   hipEvent_t prof_ev[2 * (MAX_ROUNDS + 1)] = {};
+  hipEvent_t prof_fin[MAX_ROUNDS] = {};  // after round j's finalize
 };
 
 namespace {
@@ -485,7 +486,10 @@ int enqueue_rounds(sc_proof* p, int mode) {
       SC_HIP(p->ln->pass(s, p->sid, p->tmax, p->pmax, a, (unsigned)g));
     }
     if (p->profiling) SC_HIP(hipEventRecord(p->prof_ev[2 * j + 1], s));
-    if (p->fused_finalize && !wide) continue;
+    if (p->fused_finalize && !wide) {
+      if (p->profiling) SC_HIP(hipEventRecord(p->prof_fin[j], s));
+      continue;
+    }
     FinalizeArgs f;
     f.partials = p->d_partials;
     f.nblocks = (uint32_t)g;
@@ -499,6 +503,7 @@ int enqueue_rounds(sc_proof* p, int mode) {
     f.scale_pow = round0_scale_pow(p, j, wide);
     f.pad0 = 0;
     SC_HIP(p->ln->finalize(s, p->pmax, f));
+    if (p->profiling) SC_HIP(hipEventRecord(p->prof_fin[j], s));
   }
   if (jt < p->num_vars) {
     TailArgs t;
@@ -581,6 +586,8 @@ void free_proof(sc_proof* p) {
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
   for (hipEvent_t e : p->prof_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : p->prof_fin)
     if (e) (void)hipEventDestroy(e);
   delete p;
 }
@@ -972,6 +979,7 @@ int sc_proof_set_profiling(sc_proof* p, int32_t enable) {
     // timestamps only: without the system-scope fence of a default event, the records
     // between rounds cost ~1.5% of a 2^26-row proof instead of ~4%
     for (hipEvent_t& e : p->prof_ev) SC_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    for (hipEvent_t& e : p->prof_fin) SC_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
   }
   p->profiling = enable != 0;
   return SC_OK;
@@ -991,6 +999,18 @@ int sc_proof_kernel_times(sc_proof* p, float* pass_ms, int32_t capacity, float* 
   if (tail_ms) {
     *tail_ms = 0.0f;
     if (jt < p->num_vars) SC_HIP(hipEventElapsedTime(tail_ms, p->prof_ev[2 * MAX_ROUNDS], p->prof_ev[2 * MAX_ROUNDS + 1]));
+  }
+  return SC_OK;
+}
+
+int sc_proof_round_times(sc_proof* p, float* pass_ms, float* finalize_ms, int32_t capacity, float* tail_ms) {
+  int rc = sc_proof_kernel_times(p, pass_ms, capacity, tail_ms);
+  if (rc != SC_OK) return rc;
+  const int jt = tail_first_round(p);
+  for (int j = 0; j < jt && j < capacity; ++j) {
+    float ms = 0.0f;
+    SC_HIP(hipEventElapsedTime(&ms, p->prof_ev[2 * j + 1], p->prof_fin[j]));
+    if (finalize_ms) finalize_ms[j] = ms;
   }
   return SC_OK;
 }

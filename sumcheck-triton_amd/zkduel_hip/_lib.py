@@ -82,6 +82,9 @@ SIGNATURES = [
     ("sc_proof_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("sc_proof_kernel_times", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
+    ("sc_proof_round_times", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_int32,
+      ctypes.POINTER(ctypes.c_float)]),
     ("sc_proof_final", ctypes.c_int, [ctypes.c_void_p, u32p, u32p]),
     ("sc_proof_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("sc_prove", ctypes.c_int,

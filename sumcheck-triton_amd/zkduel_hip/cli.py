@@ -7,7 +7,12 @@ experiments/h100/tools/validate_cuda_vs_registry.py:66-91).
     python -m zkduel_hip.cli --workloads poly_abc --bit-widths 64 --rounds 10 16 \\
         --warmups 0 --repeats 1 --challenge-mode sha3 --out results.csv
 
-Differences: --layout limb hands the generated tables over limb-major ([table][limb][entry])
+Differences: the header and rows carry six more columns after the reference's 28
+(proofs_per_s, p50_ms, p99_ms, algo_bytes, hbm_gbps, roofline_frac); with --timing-mode
+round-sum, eval_median_ms is the device time of the pass kernels (each round's fold fused in)
+and the tail, challenge_median_ms that of the finalize launches (sum of partials and device
+transcript), and fold_median_ms is empty (no separate fold launch); --layout limb hands the
+generated tables over limb-major ([table][limb][entry])
 and the library transposes them once on the device, so the kernels always stream
 element-major; --block is accepted and ignored (the pass grid is chosen per field); errors exit
 with status 2 (bad flags) or 1 (runtime), as the reference.
@@ -19,7 +24,7 @@ import csv
 import sys
 from pathlib import Path
 
-from .driver import CSV_COLUMNS, time_one
+from .driver import CSV_COLUMNS, EXTRA_COLUMNS, time_one
 from .fields import FIELD_SPECS
 from .workloads import ALIASES, SPEC_NAMES, ZK_SPEC_NAMES, canonical_workload_name
 
@@ -107,7 +112,8 @@ def main(argv: list[str] | None = None) -> int:
     if out.parent:
         out.parent.mkdir(parents=True, exist_ok=True)
     with out.open("w", newline="") as fh:
-        w = csv.DictWriter(fh, fieldnames=CSV_COLUMNS, extrasaction="ignore")
+        # the reference's 28 columns, then the throughput and roofline columns
+        w = csv.DictWriter(fh, fieldnames=CSV_COLUMNS + EXTRA_COLUMNS, extrasaction="ignore")
         w.writeheader()
         for bw in sorted(FIELD_SPECS):
             if bw not in a["bit_widths"]:

@@ -28,7 +28,11 @@ CSV_COLUMNS = [
     "repeats", "timing_mode", "challenge_mode", "first_ms", "median_ms", "eval_median_ms",
     "challenge_median_ms", "fold_median_ms", "total_median_ms", "input_mib", "checksum", "status",
 ]
-EXTRA_COLUMNS = ["proofs_per_s", "p50_ms", "p99_ms", "algo_bytes", "hbm_gbps"]
+# Written after the reference's columns (joins that read rows by name ignore them): whole-proof
+# throughput and latency percentiles over the repeats, the algorithmic HBM bytes of one proof,
+# the rate they are moved at, and that rate over the 8 TB/s peak.
+EXTRA_COLUMNS = ["proofs_per_s", "p50_ms", "p99_ms", "algo_bytes", "hbm_gbps", "roofline_frac"]
+HBM_PEAK_GBPS = 8000.0
 
 
 def algorithmic_bytes(num_tables: int, limbs: int, num_vars: int) -> int:
@@ -47,6 +51,16 @@ def run_timed(proof: Proof, challenge_mode: str, timer: str = "host", challenges
     dev_ms = proof.wait()
     host_ms = (time.perf_counter() - t0) * 1000.0
     return dev_ms if timer == "events" else host_ms
+
+
+def phase_ms(proof: Proof) -> tuple[float, float]:
+    """(eval, challenge) device ms of the last profiled run, the reference's round-sum phases
+    (field_sumcheck.cu:1031-1084): eval = every round's pass launches, whose fold of the
+    previous round is fused in (point-parallel rounds: their fold and evaluation launches),
+    plus the tail kernel, which runs the last rounds' eval, fold and challenge in one launch;
+    challenge = the finalize launches (partials summed, g_j stored, r_j derived)."""
+    passes, fins, tail = proof.round_times()
+    return sum(passes) + tail, sum(fins)
 
 
 def time_one(
@@ -87,22 +101,35 @@ def time_one(
         source = {"input": "host-mont-limb", "data": limb_major}
     else:
         source = {"input": input_mode, "seed": in_seed}
+    fused = timing_mode == "round-sum"
     with Proof(ctx, bit_width, comp, rounds, **source) as proof:
+        if fused:
+            proof.set_profiling(True)
         first_ms = run_timed(proof, challenge_mode, timer)
         _p, _c, checksum = proof.result()
         for _ in range(warmups):
             run_timed(proof, challenge_mode, timer)
             checksum += proof.result()[2]
-        samples = []
+        samples, evals, challenges = [], [], []
         for _ in range(repeats):
             samples.append(run_timed(proof, challenge_mode, timer))
             checksum += proof.result()[2]
+            if fused:
+                e, c = phase_ms(proof)
+                evals.append(e)
+                challenges.append(c)
         polys, chal, _cs = proof.result()
     median_ms = statistics.median(samples) if samples else first_ms
     sorted_s = sorted(samples) if samples else [first_ms]
     p99 = sorted_s[min(len(sorted_s) - 1, int(round(0.99 * (len(sorted_s) - 1))))]
     algo = algorithmic_bytes(spec.num_vars, field.limbs, rounds)
-    fused = timing_mode == "round-sum"
+    if fused and evals:
+        eval_col, chal_col = f"{statistics.median(evals):.3f}", f"{statistics.median(challenges):.6f}"
+        total_col = f"{statistics.median(samples):.3f}"
+        fold_col = ""  # no separate fold launch: each round's fold is fused into the next pass
+    else:  # whole-run timing: the reference writes the phase columns as zero
+        eval_col, chal_col, fold_col, total_col = "0.000", "0.000000", "0.000", f"{median_ms:.3f}"
+    gbps = algo / (median_ms * 1e6) if median_ms > 0 else float("inf")
     row = {
         "workload": spec.name,
         "polynomial": spec.polynomial,
@@ -125,12 +152,10 @@ def time_one(
         "challenge_mode": challenge_mode,
         "first_ms": f"{first_ms:.3f}",
         "median_ms": f"{median_ms:.3f}",
-        # fold, eval and challenge run fused in one pass per round, so round-sum timing
-        # reports the whole round under eval.
-        "eval_median_ms": f"{median_ms:.3f}" if fused else "0.000",
-        "challenge_median_ms": "0.000000",
-        "fold_median_ms": "0.000",
-        "total_median_ms": f"{median_ms:.3f}",
+        "eval_median_ms": eval_col,
+        "challenge_median_ms": chal_col,
+        "fold_median_ms": fold_col,
+        "total_median_ms": total_col,
         "input_mib": f"{spec.num_vars * n * field.limbs * 4 / (1024 ** 2):.6f}",
         "checksum": checksum,
         "status": "ok",
@@ -138,7 +163,8 @@ def time_one(
         "p50_ms": f"{median_ms:.3f}",
         "p99_ms": f"{p99:.3f}",
         "algo_bytes": algo,
-        "hbm_gbps": f"{algo / (median_ms * 1e6):.1f}" if median_ms > 0 else "inf",
+        "hbm_gbps": f"{gbps:.1f}",
+        "roofline_frac": f"{gbps / HBM_PEAK_GBPS:.4f}",
         "round_polys": polys,
         "challenges": chal,
     }

@@ -305,6 +305,16 @@ class Proof:
         _lib.check(self.lib.sc_proof_kernel_times(self.handle, buf, rounds, ctypes.byref(tail)))
         return [float(buf[j]) for j in range(rounds)], float(tail.value)
 
+    def round_times(self) -> tuple[list[float], list[float], float]:
+        """(pass ms per round before the tail, finalize ms per such round, tail ms) of the last
+        profiled run."""
+        rounds = self.info()["pass_rounds"]
+        passes = (ctypes.c_float * max(rounds, 1))()
+        fins = (ctypes.c_float * max(rounds, 1))()
+        tail = ctypes.c_float()
+        _lib.check(self.lib.sc_proof_round_times(self.handle, passes, fins, rounds, ctypes.byref(tail)))
+        return [float(passes[j]) for j in range(rounds)], [float(fins[j]) for j in range(rounds)], float(tail.value)
+
     def canonical_result(self) -> tuple[list[list[int]], list[int], int]:
         """Round polynomials and challenges decoded to canonical ints, plus the checksum."""
         polys, chal, cs = self.result()

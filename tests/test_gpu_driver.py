@@ -95,3 +95,29 @@ def test_limb_major_inputs_give_identical_proofs():
             hip.hipFree(dev)
         assert np.array_equal(got[0], want[0]) and got[2] == want[2]
     ctx.close()
+
+
+def test_round_sum_phase_columns_and_extra_columns(tmp_path):
+    """--timing-mode round-sum fills eval (pass kernels with the fold fused in, and the tail)
+    and challenge (finalize launches) from device events; fold has no launch of its own and
+    is empty; the six extra columns follow the reference's 28."""
+    from zkduel_hip.driver import CSV_COLUMNS, EXTRA_COLUMNS
+
+    out = tmp_path / "rs.csv"
+    rc = cli_main(["--workloads", "poly_abc_plus_de", "--bit-widths", "64", "256", "--rounds", "16",
+                   "--warmups", "1", "--repeats", "3", "--timing-mode", "round-sum", "--challenge-mode", "sha3",
+                   "--out", str(out)])
+    assert rc == 0
+    with out.open() as fh:
+        header = fh.readline().strip().split(",")
+    assert header == CSV_COLUMNS + EXTRA_COLUMNS
+    rows = list(csv.DictReader(out.open()))
+    assert len(rows) == 2
+    for row in rows:
+        ev, ch, total = float(row["eval_median_ms"]), float(row["challenge_median_ms"]), float(row["total_median_ms"])
+        assert ev > 0 and ch > 0 and ev + ch <= total * 1.05, row
+        assert row["fold_median_ms"] == ""
+        assert float(row["proofs_per_s"]) > 0 and int(row["algo_bytes"]) > 0
+        assert 0 < float(row["roofline_frac"]) < 1
+    whole = time_one("poly_abc_plus_de", 64, 12, warmups=0, repeats=2, timing_mode="whole-run")
+    assert whole["eval_median_ms"] == "0.000" and whole["fold_median_ms"] == "0.000"
