@@ -250,7 +250,15 @@ __device__ __forceinline__ void eval_points(typename Work<F>::V (&cur)[TMAX], ty
     if (t < npts) {
       table_for<TMAX, X>([&](auto vi) {
         const int v = vi;
-        if (v < T) cur[v] = W::step(cur[v], oth[v]);
+        if constexpr (SID != 0 && W::kLazyStep) {
+          if constexpr (static_lazy_step(SID, vi, (int)ti + 2)) {
+            cur[v] = W::step_lazy(cur[v], oth[v]);
+          } else {
+            cur[v] = W::step(cur[v], oth[v]);
+          }
+        } else {
+          if (v < T) cur[v] = W::step(cur[v], oth[v]);
+        }
       });
       compose<F, SID, TMAX>(acc[t], cur, cd, K);
     }
@@ -627,14 +635,16 @@ __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   // The 2^127 - 1 field loads all tables of a pair first, in one loop per round kind
   // (load_pair_fixed; a*b*c + d*e num_vars 24 on MI355X: 1.69 -> 1.63 ms, same box). The
-  // other fields keep the per-table order: grouped loads measured no faster at 32 bits and
-  // slower at 64 (Goldilocks round 1 798 -> 825 us), and the 255-bit field is bound by its
-  // registers.
+  // 255-bit field loads one table ahead of the one it converts (load_pair_pipelined): a whole
+  // pair of its tables does not fit its registers. The other fields keep the per-table order:
+  // grouped loads measured no faster at 32 bits and slower at 64 (Goldilocks round 1 798 ->
+  // 825 us).
   if (L <= 4 && !fold) {
     // Round 0 only reads: the next pair's loads are issued before this pair is evaluated,
     // so a wave's evaluation overlaps its own next memory round trip (Goldilocks num_vars 26
-    // proof 2.48 -> 2.45 ms, 2^32-5 1.38 -> 1.35 ms). The 255-bit field has no registers
-    // to spare for it.
+    // proof 2.48 -> 2.45 ms, 2^32-5 1.38 -> 1.35 ms). Unrolling by two, so the two register
+    // sets alternate instead of being copied, measured slower (Goldilocks round 0 0.53 ->
+    // 0.59 ms, same box).
     uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     typename W::V cur[TMAX], oth[TMAX];
     if (i < a.pairs) load_pair_fixed<F, TMAX, false, true>(cur, oth, a.src, a.dst, a.src_n, i, T, r);

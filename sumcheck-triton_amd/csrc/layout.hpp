@@ -78,6 +78,28 @@ constexpr bool static_terms_disjoint(int id) {
           if (STATIC_COMPS[id].vars[k][m] == STATIC_COMPS[id].vars[k2][m2]) return false;
   return true;
 }
+// Whether a compiled composition's interpolated value of table v at point t (t <= 3) may skip
+// the limb normalization in the 255-bit form: only when the table is a later factor of its
+// term (the second operand of the products it enters, which takes limbs below 2^31 when the
+// first is normalized) and, where it enters the two-product sum of the last products (whose
+// 18-product columns need limbs below 2^30), only at t = 2 (x(2) = x(1) + d stays below 2^30;
+// x(3) is normalized from it).
+constexpr bool static_lazy_step(int id, int v, int t) {
+  const StaticComp& c = STATIC_COMPS[id];
+  if (static_degree(id) > 3 || t > 3) return false;
+  const bool shared_sum = c.num_terms == 2 && c.kinds[0] == COEFF_ONE && c.kinds[1] == COEFF_ONE && c.lens[0] >= 2 &&
+                          c.lens[1] >= 2;
+  for (int k = 0; k < c.num_terms; ++k) {
+    if (c.vars[k][0] == v) return false;
+    for (int m = 1; m < c.lens[k]; ++m) {
+      if (c.vars[k][m] != v) continue;
+      const bool in_sum = shared_sum && m == c.lens[k] - 1;
+      return in_sum ? t == 2 : true;
+    }
+  }
+  return false;
+}
+
 static_assert(static_terms_disjoint(1) && static_terms_disjoint(2) && static_terms_disjoint(3),
               "the round-0 scaling reads each table exactly or scaled for all its uses");
 enum ChallengeMode : int32_t { CHALLENGE_FIXED = 0, CHALLENGE_SHA3 = 1, CHALLENGE_EXPLICIT = 2 };

@@ -33,6 +33,7 @@ struct Work {
   // The 255-bit form reads Montgomery input more cheaply in two places (Work<FieldP256>):
   // folding straight from it, and reading it scaled in round 0 of compiled compositions.
   static constexpr bool kCheapMontInput = false;
+  static constexpr bool kLazyStep = false;  // step_lazy: an interpolation step left unnormalized
   using V = Elem<L>;
   SC_HD static V load(const uint32_t* w) {
     V x;
@@ -305,6 +306,7 @@ struct Work<FieldP256> {
   static constexpr int L = 8;
   static constexpr bool kMontIsStored = false;
   static constexpr bool kCheapMontInput = true;
+  static constexpr bool kLazyStep = true;
   using V = R29;
   SC_HD static V from_mont(const uint32_t* w) { return r29_reduce(r29_unpack<5>(w)); }
   // The caller's Montgomery X = x 2^256 < p taken as the working value itself, which stands
@@ -354,6 +356,14 @@ struct Work<FieldP256> {
 #pragma unroll
     for (int i = 0; i < 9; ++i) s.l[i] = x.l[i] + d.l[i];
     return r29_normalize(s);
+  }
+  // x + d with the limbs left as summed (x, d normalized: limbs below 2^30), for a value that
+  // only enters products as their second operand (static_lazy_step).
+  SC_HD static V step_lazy(const V& x, const V& d) {
+    R29 s;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) s.l[i] = x.l[i] + d.l[i];
+    return s;
   }
   // lo + t d for 0 <= t < 8 (limbs stay below 2^32; the value below 29p as for t steps)
   SC_HD static V at(const V& lo, const V& d, int t) {
@@ -528,6 +538,7 @@ struct Work<FieldP128> {
   static constexpr int L = 4;
   static constexpr bool kMontIsStored = false;
   static constexpr bool kCheapMontInput = false;
+  static constexpr bool kLazyStep = false;
   using V = R26;
   // Montgomery X = 2x (X < p): x is X rotated right by one within 127 bits.
   SC_HD static V from_mont(const uint32_t* w) {

@@ -47,6 +47,8 @@ extern "C" void field_host_op(int bit_width, int op, uint64_t n, const uint32_t*
 //   op 8: a + c (b - a) folded straight from the Montgomery words of a and b (fold_mont),
 //         through store and reload (fields with a cheaper Montgomery-input path; else op 1)
 //   op 9: a read scaled (from_mont_scaled: the residue times 2^-5; else from_mont)
+//   op 10: a x3 + (a x2 + c x2) with x2, x3 = x(2), x(3) left unnormalized (step_lazy), the
+//          largest second operands the compiled compositions pass to mul and mul_sum
 template <class F>
 static void run_work(int op, uint64_t n, const uint32_t* a, const uint32_t* b, const uint32_t* c, uint32_t* out) {
   using W = Work<F>;
@@ -89,6 +91,18 @@ static void run_work(int op, uint64_t n, const uint32_t* a, const uint32_t* b, c
       } else {
         acc.add(x);
       }
+    } else if (op == 10) {
+      const typename W::V d = W::diff(y, x);
+      typename W::V x2, x3;
+      if constexpr (W::kLazyStep) {
+        x2 = W::step_lazy(y, d);
+        x3 = W::step_lazy(x2, d);
+      } else {
+        x2 = W::step(y, d);
+        x3 = W::step(x2, d);
+      }
+      acc.add(W::mul(x, x3));
+      acc.add(W::mul_sum(x, x2, z, x2));
     } else if (op == 6 || op == 7) {
       // a single-factor term with coefficient -1 is the first term a thread adds: the
       // subtraction must hold for the largest interpolated value on its own

@@ -149,6 +149,11 @@ def test_host_working_form(harness, bw):
     assert harness.work(bw, 8, a, b, c) == [(x + mont(z, y - x)) % p for x, y, z in zip(a, b, c)]
     inv32 = pow(32, -1, p) if bw == 256 else 1
     assert harness.work(bw, 9, a, b, c) == [x * inv32 % p for x in a]
+    # products whose second operands are interpolated values left unnormalized
+    x2s = [(x + 2 * (y - x)) % p for x, y in zip(a, b)]
+    x3s = [(x + 3 * (y - x)) % p for x, y in zip(a, b)]
+    assert harness.work(bw, 10, a, b, c) == [(mont(x, u3) + mont(x, u2) + mont(z, u2)) % p
+                                             for x, z, u2, u3 in zip(a, c, x2s, x3s)]
 
 
 def test_host_goldilocks_product_sum_one_reduction(harness):
