@@ -345,10 +345,12 @@ __device__ __forceinline__ void load_pair_fixed(typename Work<F>::V (&cur)[TMAX]
 
 // The 255-bit field's per-pair loads: table v + 1's words are requested before table v is
 // converted and folded (two tables' words in flight per thread; all of them would not fit the
-// registers). FOLD / SRC_MONT fix the round kind at compile time: round 0 (SRC_MONT, no fold),
-// round 1 (folding the caller's Montgomery input) or a later round (folding stored entries).
-// SCALED_EXACT (round 0 of a compiled composition): table v is read as from_mont_scaled
-// unless SCALED_EXACT says it is one of the factors read exactly (static_round0_exact).
+// registers). Requesting the next pair's first table before this pair's evaluation, so no
+// pair starts on a whole memory round trip, measured slower (254 VGPRs; a*b*c + d*e num_vars
+// 24 folding rounds 4-10% slower, same box). FOLD / SRC_MONT fix the round kind at compile
+// time: round 0 (SRC_MONT, no fold), round 1 (folding the caller's Montgomery input) or a
+// later round (folding stored entries). Round 0 of a compiled composition reads table v as
+// from_mont_scaled unless it is one of the factors read exactly (static_round0_exact).
 // r32 = Work::times32(r) for round 1.
 template <class F, int TMAX, bool FOLD, bool SRC_MONT, int SID>
 __device__ __forceinline__ void load_pair_pipelined(typename Work<F>::V (&cur)[TMAX], typename Work<F>::V (&oth)[TMAX],

@@ -178,6 +178,11 @@ SC_HD R29 r29_reduce(const R29& x) {
 // Montgomery product a * b * 2^-261 mod p by product scanning (column k of a*b and of m*p
 // summed in 64 bits; m_k clears the column's low 29 bits). p = 1 (mod 2^29), so
 // -p^-1 = -1 and m_k = -column mod 2^29.
+//
+// Each of the low nine columns is summed with 2^29 - 1 added (s = column + 2^29 - 1): the
+// column plus m_k is a multiple of 2^29, so its carry (column + m_k) / 2^29 is the column
+// over 2^29 rounded up, which is s >> 29; and -column = -(s + 1) = ~s (mod 2^29) gives m_k.
+// This saves the negation's separate add of m_k (and the zero upper word it needs) per column.
 SC_HD R29 r29_mul(const R29& a, const R29& b) {
   uint32_t m[9];
   R29 r;
@@ -186,13 +191,13 @@ SC_HD R29 r29_mul(const R29& a, const R29& b) {
   // it); m_k * p_0 = m_k since p_0 = 1.
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    uint64_t s = acc;
+    uint64_t s = acc + r29::MASK;
 #pragma unroll
     for (int i = 0; i <= k; ++i) s += (uint64_t)a.l[i] * b.l[k - i];
 #pragma unroll
     for (int i = 0; i < k; ++i) s += (uint64_t)m[i] * r29::P(k - i);
-    m[k] = (0u - (uint32_t)s) & r29::MASK;
-    acc = (s + m[k]) >> 29;
+    m[k] = ~(uint32_t)s & r29::MASK;
+    acc = s >> 29;
   }
 #pragma unroll
   for (int k = 9; k < 17; ++k) {
@@ -218,7 +223,7 @@ SC_HD R29 r29_mul_sum(const R29& a1, const R29& b1, const R29& a2, const R29& b2
   uint64_t acc = 0;
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    uint64_t s = acc;
+    uint64_t s = acc + r29::MASK;  // carry and m_k as in r29_mul
 #pragma unroll
     for (int i = 0; i <= k; ++i) {
       s += (uint64_t)a1.l[i] * b1.l[k - i];
@@ -226,8 +231,8 @@ SC_HD R29 r29_mul_sum(const R29& a1, const R29& b1, const R29& a2, const R29& b2
     }
 #pragma unroll
     for (int i = 0; i < k; ++i) s += (uint64_t)m[i] * r29::P(k - i);
-    m[k] = (0u - (uint32_t)s) & r29::MASK;
-    acc = (s + m[k]) >> 29;
+    m[k] = ~(uint32_t)s & r29::MASK;
+    acc = s >> 29;
   }
 #pragma unroll
   for (int k = 9; k < 17; ++k) {
