@@ -16,6 +16,9 @@ proof replayed as one captured hipGraph (`graph_ms_per_proof`), timed separately
 (src/zkduel/reference.py:44-110) on a bounded sample and scales it to the full size.
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu] [--widths 64 256]
+                       [--sections widths,abc16,cfg3,batched,serving,cpu]
+(--sections picks the parts to run, for profiling one of them; a run without 'widths' prints
+only the parts it ran, not the benchmark line.)
 """
 
 from __future__ import annotations
@@ -36,6 +39,7 @@ HBM_PEAK_GBPS = 8000.0
 SIZES = {32: 26, 64: 26, 128: 24, 256: 24}
 WORKLOAD = "poly_abc_plus_de"
 SEED = 2026
+SECTIONS = ("widths", "abc16", "cfg3", "batched", "serving", "cpu")
 
 
 def log(*a):
@@ -353,6 +357,64 @@ def traffic_bytes(pass_rounds: int):
     return d.get("traffic_bytes_per_launch")
 
 
+def _rounded(d: dict | None) -> dict | None:
+    return None if d is None else {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
+
+
+def measure_configs(ctx, sections: set, steps: int) -> tuple[dict | None, dict | None]:
+    """The Goldilocks a*b*c num_vars 16 proof and the BLS12-381 constraint workload, each when
+    its section is selected."""
+    from zkduel_hip import Composition, get_spec
+    from zkduel_hip.synthetic import constraint_composition
+
+    cfg1 = cfg3 = None
+    if "abc16" in sections:
+        log("bench: Goldilocks a*b*c, num_vars 16")
+        cfg1 = measure_plain(ctx, 64, Composition.from_spec(get_spec("poly_abc")), 16, max(5, steps))
+    if "cfg3" in sections:
+        log("bench: BLS12-381, 10 + 2 selector tables, 12 monomials of degree 3-6, num_vars 22")
+        comp3, mask3 = constraint_composition(256)
+        cfg3 = measure_plain(ctx, 256, comp3, 22, 2, selector_mask=mask3)
+    return cfg1, cfg3
+
+
+def measure_batched_and_serving(ctx, sections: set) -> tuple[dict, dict]:
+    """The 256-instance batched mixes (64- and 255-bit) and the two Poisson serving replays,
+    each when its section is selected. Served requests get the batched mixes' tuned class
+    schedules when the batched section ran."""
+    batched = {}
+    if "batched" in sections:
+        for bw in (64, 256):
+            log(f"bench: 256 uneven instances batched vs plain, {bw}-bit")
+            batched[str(bw)] = measure_batched(ctx, bw)
+            log(f"bench:   {batched[str(bw)]['batched_proofs_per_s']:.0f} vs {batched[str(bw)]['plain_proofs_per_s']:.0f} proofs/s")
+    serving = {}
+    if "serving" in sections:
+        schedules = {c: v for r in batched.values() for c, v in r["tuned_classes"].items()}
+        for rate, n in ((300.0, 600), (5000.0, 600)):
+            log(f"bench: serving, Poisson arrivals at {rate:.0f}/s, {n} requests")
+            serving[f"rate_{rate:.0f}_per_s"] = measure_serving(ctx, rate, n, schedules=schedules or None)
+            r = serving[f"rate_{rate:.0f}_per_s"]
+            log(f"bench:   {r['proofs_per_s']:.0f} proofs/s, p50 {r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms")
+    return batched, serving
+
+
+def run_sections_only(ctx, sections: set, args) -> int:
+    """A run without the per-width proofs (a profiling run of one part): prints one JSON line
+    holding only the sections run; it is not the benchmark line."""
+    cfg1, cfg3 = measure_configs(ctx, sections, args.steps)
+    batched, serving = measure_batched_and_serving(ctx, sections)
+    out = {"partial_run": sorted(sections), "goldilocks_abc_nv16": _rounded(cfg1),
+           "bls_constraint_12_monomials_nv22": _rounded(cfg3),
+           "batched_256_uneven_vs_plain": {bw: _rounded(r) for bw, r in batched.items()},
+           "serving_poisson_mixed_widths": {k: _rounded(r) for k, r in serving.items()}}
+    if "cpu" in sections:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -367,12 +429,24 @@ def main() -> int:
     ap.add_argument("--point-parallel", type=int, default=-1,
                     help="with --pass-blocks/--tail-pairs: rounds with at most this many pairs run "
                          "point-parallel (-1: the library default)")
+    ap.add_argument("--sections", default=",".join(SECTIONS),
+                    help="comma-separated subset of " + ",".join(SECTIONS) + " (for profiling one part; "
+                         "without 'widths' the printed line holds only the sections run, not the benchmark line)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(independent proofs shard across GPUs with no communication between them)", file=sys.stderr)
         return 2
-    if 64 not in args.widths:
+    sections = {s.strip() for s in args.sections.split(",") if s.strip()}
+    unknown = sections - set(SECTIONS)
+    if unknown:
+        print(f"bench.py: unknown sections {sorted(unknown)}; choose from {','.join(SECTIONS)}", file=sys.stderr)
+        return 2
+    if args.no_cpu:
+        sections.discard("cpu")
+    if args.no_batch:
+        sections -= {"batched", "serving"}
+    if "widths" in sections and 64 not in args.widths:
         args.widths = [64] + args.widths
 
     from zkduel_hip import Context
@@ -380,6 +454,8 @@ def main() -> int:
     ctx = Context(0)
     calib = ctx.calibrate()
     log("bench: calibration " + json.dumps({k: f"{v:.4g}" for k, v in calib.items()}))
+    if "widths" not in sections:
+        return run_sections_only(ctx, sections, args)
     per = {}
     for bw in args.widths:
         steps = args.steps if bw == 64 else max(3, args.steps // 2)
@@ -390,31 +466,10 @@ def main() -> int:
         log(f"bench: width {bw}: {per[bw]['proofs_per_s']:.1f} proofs/s, "
             f"pass kernel {per[bw]['pass_kernel_gbps']:.0f} GB/s")
     head = per[64]
-    from zkduel_hip import Composition, get_spec
-    from zkduel_hip.synthetic import constraint_composition
-
-    log("bench: Goldilocks a*b*c, num_vars 16")
-    cfg1 = measure_plain(ctx, 64, Composition.from_spec(get_spec("poly_abc")), 16, max(5, args.steps))
-    log("bench: BLS12-381, 10 + 2 selector tables, 12 monomials of degree 3-6, num_vars 22")
-    comp3, mask3 = constraint_composition(256)
-    cfg3 = measure_plain(ctx, 256, comp3, 22, 2, selector_mask=mask3)
-    batched = {}
-    if not args.no_batch:
-        for bw in (64, 256):
-            log(f"bench: 256 uneven instances batched vs plain, {bw}-bit")
-            batched[str(bw)] = measure_batched(ctx, bw)
-            log(f"bench:   {batched[str(bw)]['batched_proofs_per_s']:.0f} vs {batched[str(bw)]['plain_proofs_per_s']:.0f} proofs/s")
-    serving = {}
-    if not args.no_batch:
-        # the batched mix's tuned class schedules, applied to served requests of those classes
-        schedules = {c: v for r in batched.values() for c, v in r["tuned_classes"].items()}
-        for rate, n in ((300.0, 600), (5000.0, 600)):
-            log(f"bench: serving, Poisson arrivals at {rate:.0f}/s, {n} requests")
-            serving[f"rate_{rate:.0f}_per_s"] = measure_serving(ctx, rate, n, schedules=schedules)
-            r = serving[f"rate_{rate:.0f}_per_s"]
-            log(f"bench:   {r['proofs_per_s']:.0f} proofs/s, p50 {r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms")
+    cfg1, cfg3 = measure_configs(ctx, sections, args.steps)
+    batched, serving = measure_batched_and_serving(ctx, sections)
     cpu = None
-    if not args.no_cpu:
+    if "cpu" in sections:
         log("bench: cpu baseline")
         cpu = cpu_baseline()
     line = {
@@ -439,16 +494,11 @@ def main() -> int:
             "parallelism": "single GPU",
             "p50_ms": round(head["p50_ms"], 4),
             "p99_ms": round(head["p99_ms"], 4),
-            "per_width": {
-                str(bw): {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
-                for bw, r in per.items()
-            },
-            "goldilocks_abc_nv16": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in cfg1.items()},
-            "bls_constraint_12_monomials_nv22": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in cfg3.items()},
-            "batched_256_uneven_vs_plain": {bw: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
-                                         for bw, r in batched.items()},
-            "serving_poisson_mixed_widths": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in r.items()}
-                                             for k, r in serving.items()},
+            "per_width": {str(bw): _rounded(r) for bw, r in per.items()},
+            "goldilocks_abc_nv16": _rounded(cfg1),
+            "bls_constraint_12_monomials_nv22": _rounded(cfg3),
+            "batched_256_uneven_vs_plain": {bw: _rounded(r) for bw, r in batched.items()},
+            "serving_poisson_mixed_widths": {k: _rounded(r) for k, r in serving.items()},
             "calibration": {
                 "mad_u64_u32_lane_ops_per_s": calib["mad_u64_u32"],
                 "hbm_read_gbps": calib["hbm_read"] / 1e9,
