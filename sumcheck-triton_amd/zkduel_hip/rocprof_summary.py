@@ -42,7 +42,7 @@ def pmc_per_kernel(db: sqlite3.Connection, counter: str) -> dict[str, tuple[int,
     return {k: (len(v), sum(v) / len(v)) for k, v in acc.items()}
 
 
-RESOURCE_COLUMNS = ("arch_vgpr_count", "accum_vgpr_count", "sgpr_count", "lds_size", "static_lds_size",
+RESOURCE_COLUMNS = ("vgpr_count", "arch_vgpr_count", "accum_vgpr_count", "sgpr_count", "lds_size", "static_lds_size",
                     "group_segment_size", "scratch_size", "static_scratch_size", "private_segment_size")
 
 
