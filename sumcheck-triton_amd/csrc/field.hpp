@@ -369,11 +369,14 @@ struct Accum<FieldP64> {
 // acc (64-bit) + top (32-bit) += a * b. On the device this is v_mad_u64_u32 with its carry
 // out, then v_addc_co_u32 into top: two instructions per multiply-accumulate, with the
 // accumulator pair used in place as the 64-bit addend (a C++ formulation makes the compiler
-// re-pair zero-extended halves with extra moves). Arithmetic only, no memory operands.
+// re-pair zero-extended halves with extra moves). Arithmetic only, no memory operands. The
+// s_nop gives the two wait states gfx950 needs between a VALU write of an SGPR (the carry)
+// and a VALU read of it, which the compiler does not insert inside an asm statement.
 SC_HD void mac(uint64_t& acc, uint32_t& top, uint32_t a, uint32_t b) {
 #if defined(__HIP_DEVICE_COMPILE__)
   uint64_t carry;
   asm("v_mad_u64_u32 %0, %1, %3, %4, %0\n\t"
+      "s_nop 1\n\t"
       "v_addc_co_u32_e64 %2, %1, 0, %2, %1"
       : "+v"(acc), "=&s"(carry), "+v"(top)
       : "v"(a), "v"(b));

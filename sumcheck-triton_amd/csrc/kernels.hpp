@@ -268,12 +268,13 @@ __device__ __forceinline__ void eval_points(typename Work<F>::V (&cur)[TMAX], ty
 // Loads pair i of the current round for every table, folding from the previous round's
 // tables when fold != 0 (f' = e + r (o - e), field_sumcheck.cu:677-694), and writes the
 // folded entries to dst. src holds the caller's Montgomery tables when src_mont (rounds 0
-// and 1), else entries written by an earlier fold.
+// and 1), else entries written by an earlier fold. Table v of src starts at entry v * src_st
+// (src_n for fold buffers, the input's stride for the input); dst tables are packed.
 template <class F, int TMAX>
 __device__ __forceinline__ void load_pair(typename Work<F>::V (&cur)[TMAX], typename Work<F>::V (&oth)[TMAX],
                                           const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
-                                          uint64_t src_n, uint64_t i, int T, bool fold, bool src_mont,
-                                          const typename Work<F>::V& r) {
+                                          uint64_t src_n, uint64_t src_st, uint64_t i, int T, bool fold,
+                                          bool src_mont, const typename Work<F>::V& r) {
   using W = Work<F>;
   constexpr int L = F::L;
   table_for<TMAX, (TMAX <= 8)>([&](auto vi) {
@@ -281,7 +282,7 @@ __device__ __forceinline__ void load_pair(typename Work<F>::V (&cur)[TMAX], type
     if (v < T) {
       if (fold) {
         uint32_t w[4 * L];
-        load_words<4 * L>(w, src + ((uint64_t)v * src_n + 4 * i) * L);
+        load_words<4 * L>(w, src + ((uint64_t)v * src_st + 4 * i) * L);
         typename W::V f[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) f[k] = src_mont ? W::from_mont(w + k * L) : W::from_stored(w + k * L);
@@ -295,7 +296,7 @@ __device__ __forceinline__ void load_pair(typename Work<F>::V (&cur)[TMAX], type
         oth[v] = o;
       } else {
         uint32_t w[2 * L];
-        load_words<2 * L>(w, src + ((uint64_t)v * src_n + 2 * i) * L);
+        load_words<2 * L>(w, src + ((uint64_t)v * src_st + 2 * i) * L);
         cur[v] = W::from_mont(w);
         oth[v] = W::from_mont(w + L);
       }
@@ -311,14 +312,15 @@ __device__ __forceinline__ void load_pair(typename Work<F>::V (&cur)[TMAX], type
 template <class F, int TMAX, bool FOLD, bool SRC_MONT>
 __device__ __forceinline__ void load_pair_fixed(typename Work<F>::V (&cur)[TMAX], typename Work<F>::V (&oth)[TMAX],
                                                 const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
-                                                uint64_t src_n, uint64_t i, int T, const typename Work<F>::V& r) {
+                                                uint64_t src_n, uint64_t src_st, uint64_t i, int T,
+                                                const typename Work<F>::V& r) {
   using W = Work<F>;
   constexpr int L = F::L;
   constexpr int NW = FOLD ? 4 * L : 2 * L;
   uint32_t w[TMAX][NW];
   table_for<TMAX, (TMAX <= 8)>([&](auto vi) {
     const int v = vi;
-    if (v < T) load_words<NW>(w[v], src + ((uint64_t)v * src_n + (FOLD ? 4 : 2) * i) * L);
+    if (v < T) load_words<NW>(w[v], src + ((uint64_t)v * src_st + (FOLD ? 4 : 2) * i) * L);
   });
   table_for<TMAX, (TMAX <= 8)>([&](auto vi) {
     const int v = vi;
@@ -355,8 +357,8 @@ __device__ __forceinline__ void load_pair_fixed(typename Work<F>::V (&cur)[TMAX]
 template <class F, int TMAX, bool FOLD, bool SRC_MONT, int SID>
 __device__ __forceinline__ void load_pair_pipelined(typename Work<F>::V (&cur)[TMAX], typename Work<F>::V (&oth)[TMAX],
                                                     const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
-                                                    uint64_t src_n, uint64_t i, int T, const typename Work<F>::V& r,
-                                                    const typename Work<F>::V& r32) {
+                                                    uint64_t src_n, uint64_t src_st, uint64_t i, int T,
+                                                    const typename Work<F>::V& r, const typename Work<F>::V& r32) {
   using W = Work<F>;
   constexpr int L = F::L;
   constexpr int NW = FOLD ? 4 * L : 2 * L;
@@ -366,7 +368,7 @@ __device__ __forceinline__ void load_pair_pipelined(typename Work<F>::V (&cur)[T
     const int v = vi;
     if (v < T) {
       if (v + 1 < T && v + 1 < TMAX)
-        load_words<NW>(w[(v + 1) & 1], src + ((uint64_t)(v + 1) * src_n + (FOLD ? 4 : 2) * i) * L);
+        load_words<NW>(w[(v + 1) & 1], src + ((uint64_t)(v + 1) * src_st + (FOLD ? 4 : 2) * i) * L);
       const uint32_t* wv = w[v & 1];
       if constexpr (FOLD) {
         typename W::V e, o;
@@ -649,10 +651,10 @@ __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
     // 0.59 ms, same box).
     uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     typename W::V cur[TMAX], oth[TMAX];
-    if (i < a.pairs) load_pair_fixed<F, TMAX, false, true>(cur, oth, a.src, a.dst, a.src_n, i, T, r);
+    if (i < a.pairs) load_pair_fixed<F, TMAX, false, true>(cur, oth, a.src, a.dst, a.src_n, a.src_stride, i, T, r);
     for (; i < a.pairs; i += stride) {
       typename W::V ncur[TMAX], noth[TMAX];
-      if (i + stride < a.pairs) load_pair_fixed<F, TMAX, false, true>(ncur, noth, a.src, a.dst, a.src_n, i + stride, T, r);
+      if (i + stride < a.pairs) load_pair_fixed<F, TMAX, false, true>(ncur, noth, a.src, a.dst, a.src_n, a.src_stride, i + stride, T, r);
       eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts, false);
 #pragma unroll
       for (int v = 0; v < TMAX; ++v) {
@@ -665,7 +667,7 @@ __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
       constexpr bool FOLD = decltype(fold_c)::value, MONT = decltype(mont_c)::value;
       for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.pairs; i += stride) {
         typename W::V cur[TMAX], oth[TMAX];
-        load_pair_fixed<F, TMAX, FOLD, MONT>(cur, oth, a.src, a.dst, a.src_n, i, T, r);
+        load_pair_fixed<F, TMAX, FOLD, MONT>(cur, oth, a.src, a.dst, a.src_n, a.src_stride, i, T, r);
         eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts, FOLD);
       }
     };
@@ -682,7 +684,7 @@ __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
       constexpr bool FOLD = decltype(fold_c)::value, MONT = decltype(mont_c)::value;
       for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.pairs; i += stride) {
         typename W::V cur[TMAX], oth[TMAX];
-        load_pair_pipelined<F, TMAX, FOLD, MONT, SID>(cur, oth, a.src, a.dst, a.src_n, i, T, r, r32);
+        load_pair_pipelined<F, TMAX, FOLD, MONT, SID>(cur, oth, a.src, a.dst, a.src_n, a.src_stride, i, T, r, r32);
         eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts, FOLD);
       }
     };
@@ -696,7 +698,7 @@ __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
   } else {
     for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.pairs; i += stride) {
       typename W::V cur[TMAX], oth[TMAX];
-      load_pair<F, TMAX>(cur, oth, a.src, a.dst, a.src_n, i, T, fold, src_mont, r);
+      load_pair<F, TMAX>(cur, oth, a.src, a.dst, a.src_n, a.src_stride, i, T, fold, src_mont, r);
       eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts, fold);
     }
   }
@@ -759,6 +761,7 @@ __device__ __forceinline__ void tail_rounds(const TailArgs& a) {
     const uint32_t* src = j <= 1 ? a.input : (((j - 1) & 1) ? a.buf_a : a.buf_b);
     uint32_t* dst = (j & 1) ? a.buf_a : a.buf_b;
     const uint64_t src_n = fold ? (a.n0 >> (j - 1)) : a.n0;
+    const uint64_t src_st = j <= 1 ? a.in_stride : src_n;
     const uint64_t pairs = a.n0 >> (j + 1);
     using W = Work<F>;
     const typename W::V r = W::from_mont(s_r);
@@ -767,7 +770,7 @@ __device__ __forceinline__ void tail_rounds(const TailArgs& a) {
     for (int t = 0; t < PMAX; ++t) acc[t].clear();
     for (uint64_t i = threadIdx.x; i < pairs; i += TB) {
       typename W::V cur[TMAX], oth[TMAX];
-      load_pair<F, TMAX>(cur, oth, src, dst, src_n, i, T, fold, j <= 1, r);
+      load_pair<F, TMAX>(cur, oth, src, dst, src_n, src_st, i, T, fold, j <= 1, r);
       eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts, fold);
     }
     Elem<L> sums[PMAX];
@@ -787,8 +790,8 @@ __device__ __forceinline__ void tail_rounds(const TailArgs& a) {
     const int v = threadIdx.x;
     typename W::V lo, hi;
     if (jl == 0) {
-      lo = W::from_mont(a.input + (uint64_t)v * 2 * L);
-      hi = W::from_mont(a.input + ((uint64_t)v * 2 + 1) * L);
+      lo = W::from_mont(a.input + (uint64_t)v * a.in_stride * L);
+      hi = W::from_mont(a.input + ((uint64_t)v * a.in_stride + 1) * L);
     } else {
       const uint32_t* last = (jl & 1) ? a.buf_a : a.buf_b;
       lo = W::from_stored(last + (uint64_t)v * 2 * L);
@@ -822,8 +825,9 @@ __global__ void __launch_bounds__(TB) tail_kernel(TailArgs a) {
 // entries, idx = table * n_new + entry), each folded from two entries of round j-1's src.
 template <class F>
 __device__ __forceinline__ void wide_fold_range(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
-                                                uint64_t src_n, bool src_mont, const typename Work<F>::V& r,
-                                                uint64_t begin, uint64_t end, uint64_t stride) {
+                                                uint64_t src_n, uint64_t src_st, bool src_mont,
+                                                const typename Work<F>::V& r, uint64_t begin, uint64_t end,
+                                                uint64_t stride) {
   using W = Work<F>;
   constexpr int L = F::L;
   const uint64_t n_new = src_n >> 1;  // a power of two
@@ -831,7 +835,7 @@ __device__ __forceinline__ void wide_fold_range(const uint32_t* __restrict__ src
   for (uint64_t idx = begin; idx < end; idx += stride) {
     const uint64_t v = idx >> shift, i = idx & (n_new - 1);
     uint32_t w[2 * L];
-    load_words<2 * L>(w, src + (v * src_n + 2 * i) * L);
+    load_words<2 * L>(w, src + (v * src_st + 2 * i) * L);
     const typename W::V lo = src_mont ? W::from_mont(w) : W::from_stored(w);
     const typename W::V hi = src_mont ? W::from_mont(w + L) : W::from_stored(w + L);
     uint32_t y[L];
@@ -909,7 +913,7 @@ __device__ __forceinline__ void compose_slots(typename Work<F>::Acc& acc, const 
 // idle). Writes the block's sums for t < npts to part[t * L + k] (zero for t = 1 from round
 // 1 on, which the round's finish derives from the claim). s_red: BLOCK * L words.
 template <class F, int TMAX>
-__device__ __forceinline__ void wide_eval_block(const uint32_t* __restrict__ src, uint64_t src_n, bool src_mont,
+__device__ __forceinline__ void wide_eval_block(const uint32_t* __restrict__ src, uint64_t src_st, bool src_mont,
                                                 int round, uint64_t begin, uint64_t end, uint64_t stride,
                                                 const CompDesc* __restrict__ cd, uint32_t* __restrict__ part,
                                                 uint32_t* s_red) {
@@ -932,7 +936,7 @@ __device__ __forceinline__ void wide_eval_block(const uint32_t* __restrict__ src
         constexpr int v = decltype(vi)::value;
         if (v < T) {
           uint32_t w[2 * L];
-          load_words<2 * L>(w, src + ((uint64_t)v * src_n + 2 * i) * L);
+          load_words<2 * L>(w, src + ((uint64_t)v * src_st + 2 * i) * L);
           const typename W::V lo = src_mont ? W::from_mont(w) : W::from_stored(w);
           const typename W::V hi = src_mont ? W::from_mont(w + L) : W::from_stored(w + L);
           x[v] = W::at(lo, W::diff(hi, lo), t);
@@ -978,7 +982,7 @@ __device__ __forceinline__ void wide_eval_block(const uint32_t* __restrict__ src
 template <class F>
 __global__ void __launch_bounds__(BLOCK) wide_fold_kernel(PassArgs a) {
   const uint64_t total = (a.src_n >> 1) * (uint64_t)a.comp->num_tables;
-  wide_fold_range<F>(a.src, a.dst, a.src_n, a.round <= 1, Work<F>::from_mont(a.st->r[a.round - 1]),
+  wide_fold_range<F>(a.src, a.dst, a.src_n, a.src_stride, a.round <= 1, Work<F>::from_mont(a.st->r[a.round - 1]),
                      (uint64_t)blockIdx.x * BLOCK + threadIdx.x, total, (uint64_t)gridDim.x * BLOCK);
 }
 
@@ -986,7 +990,7 @@ template <class F, int TMAX, int G>
 __global__ void __launch_bounds__(BLOCK) wide_eval_kernel(PassArgs a) {
   __shared__ uint32_t s_red[BLOCK * F::L];
   const uint64_t pairs_here = wide_pairs_per_block(a.comp->npts, a.round);
-  wide_eval_block<F, TMAX>(a.src, a.src_n, a.round == 0, a.round, (uint64_t)blockIdx.x * pairs_here, a.pairs,
+  wide_eval_block<F, TMAX>(a.src, a.src_stride, a.round == 0, a.round, (uint64_t)blockIdx.x * pairs_here, a.pairs,
                            (uint64_t)gridDim.x * pairs_here, a.comp,
                            a.partials + (uint64_t)blockIdx.x * MAX_POINTS * F::L, s_red);
 }
@@ -1037,7 +1041,7 @@ __global__ void __launch_bounds__(BLOCK) batch_pass_kernel(BatchArgs a) {
     const uint64_t i = base + (uint64_t)it * BLOCK + threadIdx.x;
     if (i < pairs) {
       typename W::V cur[TMAX], oth[TMAX];
-      load_pair<F, TMAX>(cur, oth, src, dst, src_n, i, T, fold, j <= 1, r);
+      load_pair<F, TMAX>(cur, oth, src, dst, src_n, src_n, i, T, fold, j <= 1, r);
       eval_points<F, TMAX, PMAX, 0>(cur, oth, acc, cd, T, K, npts, fold);
     }
   }
@@ -1064,7 +1068,7 @@ __global__ void __launch_bounds__(BLOCK) batch_wide_fold_kernel(BatchArgs a) {
   const uint64_t total = (src_n >> 1) * (uint64_t)in.comp->num_tables;
   const uint64_t span = (uint64_t)BLOCK * ent.items;
   const uint64_t begin = (uint64_t)(blockIdx.x - ent.tile_begin) * span;
-  wide_fold_range<F>(src, dst, src_n, j <= 1, Work<F>::from_mont(in.st->r[j - 1]), begin + threadIdx.x,
+  wide_fold_range<F>(src, dst, src_n, src_n, j <= 1, Work<F>::from_mont(in.st->r[j - 1]), begin + threadIdx.x,
                      begin + span < total ? begin + span : total, BLOCK);
 }
 
@@ -1097,6 +1101,7 @@ __global__ void __launch_bounds__(BLOCK) batch_tail_kernel(BatchArgs a) {
   t.buf_a = in.buf_a;
   t.buf_b = in.buf_b;
   t.n0 = in.n0;
+  t.in_stride = in.n0;  // batch and server inputs are packed
   t.first_round = (int32_t)ent.round;
   t.num_rounds = in.num_vars;
   t.challenge_mode = in.challenge_mode;
@@ -1145,6 +1150,7 @@ __global__ void __launch_bounds__(BLOCK) fill_kernel(FillArgs a) {
        idx += (uint64_t)gridDim.x * BLOCK) {
     const uint32_t v = (uint32_t)(idx / a.n);
     const uint64_t o = idx - (uint64_t)v * a.n;
+    const uint64_t at = (uint64_t)v * a.stride + o;  // entry o of table v
     Elem<L> x;
     if (a.kind == FILL_AFFINE) {
       x = affine_element<F>(a.seed, v, o);
@@ -1155,7 +1161,7 @@ __global__ void __launch_bounds__(BLOCK) fill_kernel(FillArgs a) {
     }
     const Elem<L> m = mont_mul<F>(x, r2);
 #pragma unroll
-    for (int k = 0; k < L; ++k) a.tables[idx * L + k] = m.v[k];
+    for (int k = 0; k < L; ++k) a.tables[at * L + k] = m.v[k];
   }
 }
 
@@ -1164,7 +1170,7 @@ __global__ void __launch_bounds__(BLOCK) fill_kernel(FillArgs a) {
 template <class F>
 __global__ void __launch_bounds__(BLOCK) limb_to_element_kernel(const uint32_t* __restrict__ in,
                                                                 uint32_t* __restrict__ out, uint64_t n,
-                                                                int num_tables) {
+                                                                uint64_t out_stride, int num_tables) {
   constexpr int L = F::L;
   const int shift = __builtin_ctzll(n);
   const uint64_t total = n * (uint64_t)num_tables;
@@ -1175,7 +1181,7 @@ __global__ void __launch_bounds__(BLOCK) limb_to_element_kernel(const uint32_t* 
 #pragma unroll
     for (int k = 0; k < L; ++k) w[k] = in[(v * L + k) * n + i];
 #pragma unroll
-    for (int k = 0; k < L; ++k) out[idx * L + k] = w[k];
+    for (int k = 0; k < L; ++k) out[(v * out_stride + i) * L + k] = w[k];
   }
 }
 

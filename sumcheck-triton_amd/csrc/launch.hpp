@@ -29,7 +29,8 @@ struct Launchers {
   hipError_t (*wide_fold)(hipStream_t, const PassArgs&, unsigned grid);
   hipError_t (*wide_eval)(hipStream_t, int tmax, int pmax, const PassArgs&, unsigned grid);
   // limb-major input tables -> element-major (n entries per table, a power of two)
-  hipError_t (*limb_to_element)(hipStream_t, const uint32_t* in, uint32_t* out, uint64_t n, int num_tables);
+  hipError_t (*limb_to_element)(hipStream_t, const uint32_t* in, uint32_t* out, uint64_t n, uint64_t out_stride,
+                                int num_tables);
   // point-parallel form of a batched stage: fold tiles, then evaluation tiles (args.tile_base
   // = the stage's per-pair tiles, so the evaluation partials follow theirs)
   hipError_t (*batch_wide_fold)(hipStream_t, const BatchArgs&, unsigned tiles);

@@ -257,6 +257,7 @@ struct sc_proof {
   const Launchers* ln = nullptr;
   uint64_t n0 = 0;
   uint32_t* d_tables = nullptr;
+  uint64_t in_stride = 0;  // entries from one input table to the next (input_stride, or n0 for caller tables)
   bool owns_tables = false;
   uint32_t* d_a = nullptr;
   uint32_t* d_b = nullptr;
@@ -452,6 +453,7 @@ int enqueue_rounds(sc_proof* p, int mode) {
     a.src = buf_for_round(p, j == 0 ? 0 : j - 1);
     a.dst = j == 0 ? nullptr : buf_for_round(p, j);
     a.src_n = j == 0 ? p->n0 : (p->n0 >> (j - 1));
+    a.src_stride = j <= 1 ? p->in_stride : a.src_n;  // rounds 0 and 1 read the input
     a.pairs = p->n0 >> (j + 1);
     a.round = j;
     a.num_rounds = p->num_vars;
@@ -477,6 +479,7 @@ int enqueue_rounds(sc_proof* p, int mode) {
       e.src = buf_for_round(p, j);
       e.dst = nullptr;
       e.src_n = p->n0 >> j;
+      e.src_stride = j == 0 ? p->in_stride : e.src_n;
       e.fold = 0;
       e.fused_finalize = 0;
       const uint64_t pairs_per_block = wide_pairs_per_block(p->npts, j);
@@ -511,6 +514,7 @@ int enqueue_rounds(sc_proof* p, int mode) {
     t.buf_a = p->d_a;
     t.buf_b = p->d_b;
     t.n0 = p->n0;
+    t.in_stride = p->in_stride;
     t.first_round = jt;
     t.num_rounds = p->num_vars;
     t.challenge_mode = mode;
@@ -767,11 +771,14 @@ int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* co
 
   if (input->kind == SC_INPUT_DEVICE_MONT) {
     p->d_tables = (uint32_t*)input->data;
+    p->in_stride = p->n0;
     p->owns_tables = false;
   } else {
-    SC_HIP_P(alloc((void**)&p->d_tables, words * 4));
+    p->in_stride = input_stride(p->n0, p->L);
+    SC_HIP_P(alloc((void**)&p->d_tables, ((uint64_t)(p->T - 1) * p->in_stride + p->n0) * p->L * 4));
     p->owns_tables = true;
   }
+  const uint64_t table_bytes = p->n0 * p->L * 4, stride_bytes = p->in_stride * p->L * 4;
   SC_HIP_P(alloc((void**)&p->d_a, (words / 2) * 4));
   SC_HIP_P(alloc((void**)&p->d_b, (words / 4) * 4));
   SC_HIP_P(alloc((void**)&p->d_partials, (uint64_t)MAX_PASS_BLOCKS * MAX_POINTS * p->L * 4));
@@ -786,18 +793,23 @@ int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* co
 
   const FieldConsts* fc = ctx->d_fc[fidx];
   if (input->kind == SC_INPUT_HOST_CANONICAL || input->kind == SC_INPUT_HOST_MONT) {
-    SC_HIP_P(hipMemcpyAsync(p->d_tables, input->data, words * 4, hipMemcpyHostToDevice, s));
+    for (int v = 0; v < p->T; ++v) {  // one copy per table (a 2D copy's pitch is limited)
+      SC_HIP_P(hipMemcpyAsync((uint8_t*)p->d_tables + v * stride_bytes, (const uint8_t*)input->data + v * table_bytes,
+                              table_bytes, hipMemcpyHostToDevice, s));
+    }
     if (input->kind == SC_INPUT_HOST_CANONICAL) {
-      SC_HIP_P(p->ln->field_op(s, 3, (uint64_t)p->T * p->n0, p->d_tables, nullptr, p->d_tables, fc));
+      // every table in place; the gaps between tables (if any) are encoded too, unread
+      const uint64_t span = (uint64_t)(p->T - 1) * p->in_stride + p->n0;
+      SC_HIP_P(p->ln->field_op(s, 3, span, p->d_tables, nullptr, p->d_tables, fc));
     }
   } else if (input->kind == SC_INPUT_DEVICE_MONT_LIMB) {
-    SC_HIP_P(p->ln->limb_to_element(s, (const uint32_t*)input->data, p->d_tables, p->n0, p->T));
+    SC_HIP_P(p->ln->limb_to_element(s, (const uint32_t*)input->data, p->d_tables, p->n0, p->in_stride, p->T));
   } else if (input->kind == SC_INPUT_HOST_MONT_LIMB) {
     // staged through a temporary device buffer of the input's size, freed after the transpose
     uint32_t* tmp = nullptr;
     SC_HIP_P(hipMalloc(&tmp, words * 4));
     hipError_t e = hipMemcpyAsync(tmp, input->data, words * 4, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = p->ln->limb_to_element(s, tmp, p->d_tables, p->n0, p->T);
+    if (e == hipSuccess) e = p->ln->limb_to_element(s, tmp, p->d_tables, p->n0, p->in_stride, p->T);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(tmp);
     SC_HIP_P(e);
@@ -805,6 +817,7 @@ int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* co
     FillArgs f;
     f.tables = p->d_tables;
     f.n = p->n0;
+    f.stride = p->in_stride;
     f.num_tables = p->T;
     f.kind = input->kind == SC_INPUT_AFFINE ? FILL_AFFINE : FILL_UNIFORM;
     f.seed = input->seed;
@@ -949,7 +962,11 @@ int sc_proof_tables(sc_proof* p, uint32_t* out) {
   if (p == nullptr || out == nullptr) return fail(SC_ERR_ARG, "proof and out must not be NULL");
   SC_HIP(hipSetDevice(p->ctx->device));
   SC_HIP(hipStreamSynchronize(p->ctx->stream));
-  SC_HIP(hipMemcpy(out, p->d_tables, (uint64_t)p->T * p->n0 * p->L * 4, hipMemcpyDeviceToHost));
+  const uint64_t table_bytes = p->n0 * p->L * 4;
+  for (int v = 0; v < p->T; ++v) {
+    SC_HIP(hipMemcpy((uint8_t*)out + v * table_bytes, (const uint8_t*)p->d_tables + v * p->in_stride * p->L * 4,
+                     table_bytes, hipMemcpyDeviceToHost));
+  }
   return SC_OK;
 }
 
@@ -1067,6 +1084,7 @@ int sc_prove_noalloc(sc_context* ctx, int32_t bit_width, const sc_composition* c
   sc_proof p;
   init_shape(&p, ctx, fidx, desc, num_vars);
   p.d_tables = const_cast<uint32_t*>(tables);
+  p.in_stride = n0;  // the caller's tables, packed
   p.d_a = reinterpret_cast<uint32_t*>(base);
   p.d_b = p.d_a + (uint64_t)p.T * (n0 / 2) * L;
   p.d_partials = reinterpret_cast<uint32_t*>(base + NOALLOC_OFF_PARTIALS);
@@ -1403,6 +1421,7 @@ int sc_batch_create(sc_context* ctx, int32_t bit_width, int32_t n, const sc_comp
     FillArgs f;
     f.tables = (uint32_t*)base;
     f.n = b->n0[i];
+    f.stride = b->n0[i];
     f.num_tables = descs[i].num_tables;
     f.kind = inputs[i].kind == SC_INPUT_AFFINE ? FILL_AFFINE : FILL_UNIFORM;
     f.seed = inputs[i].seed;
@@ -2000,6 +2019,7 @@ int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
     FillArgs f;
     f.tables = (uint32_t*)base;
     f.n = n0;
+    f.stride = n0;
     f.num_tables = r.desc.num_tables;
     f.kind = r.input.kind == SC_INPUT_AFFINE ? FILL_AFFINE : FILL_UNIFORM;
     f.seed = r.input.seed;

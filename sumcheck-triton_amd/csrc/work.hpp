@@ -69,6 +69,222 @@ struct Work {
   using Acc = Accum<F>;
 };
 
+// ---------------------------------------------------------------------------------------------
+// Goldilocks working form: the caller's Montgomery words (R = 2^64), with products and running
+// sums built so that no 64-bit carry has to be detected.
+//
+// gl_mad(a, b, c) = a b + c for 32-bit a, b and a 64-bit c is one v_mad_u64_u32 (written as a
+// one-instruction asm statement: in C++ the backend rewrites sums of zero-extended words into
+// 64-bit adds that need extra register moves). With b = 1 it adds a 32-bit word to a 64-bit
+// one. The 128-bit product of two 64-bit words a = a1:a0, b = b1:b0 then takes 8 of them,
+// none of which can overflow:
+//   L = a0 b0,  T = a1 b0 + L1,  U = a0 b1 + T0,  V = a1 b1 + T1 + U1   (T, U < 2^64, V <= 2^64 - 1)
+//   a b = L0 + U0 2^32 + V 2^64.
+// Montgomery reduction of x = V 2^64 + x1 2^32 + x0 uses 2^96 = -1, 2^64 = 2^32 - 1 (mod p):
+//   x 2^-64 = V + x1 - (x0 + x1) 2^32 = V - D,  D = (s0 + s1) 2^32 - (x1 + s1),
+// with s = x0 + x1 = s1 2^32 + s0; 0 <= D <= 2^64 - 2^32 < p, and a borrow out of V - D is
+// paid by adding p (subtracting 2^32 - 1 from the wrapped difference). The carries of that
+// chain are __builtin_addc / __builtin_subc, so the compiler places their hazard waits.
+//
+// Value ranges: products and sums may be any 64-bit word congruent to the result (below 2^64,
+// not necessarily below p); interpolation differences, folded entries and everything stored
+// are canonical (below p), which the lazy additions x + d need of d.
+// ---------------------------------------------------------------------------------------------
+SC_HD uint32_t gl_adc(uint32_t a, uint32_t b, uint32_t ci, uint32_t& co) {
+#if defined(__clang__)
+  unsigned c;
+  const uint32_t s = __builtin_addc(a, b, ci, &c);
+  co = c;
+  return s;
+#else
+  const uint64_t s = (uint64_t)a + b + ci;
+  co = (uint32_t)(s >> 32);
+  return (uint32_t)s;
+#endif
+}
+
+SC_HD uint32_t gl_sbb(uint32_t a, uint32_t b, uint32_t bi, uint32_t& bo) {
+#if defined(__clang__)
+  unsigned c;
+  const uint32_t d = __builtin_subc(a, b, bi, &c);
+  bo = c;
+  return d;
+#else
+  const uint64_t d = (uint64_t)a - b - bi;
+  bo = (uint32_t)(d >> 63);
+  return (uint32_t)d;
+#endif
+}
+
+SC_HD uint64_t gl_mad(uint32_t a, uint32_t b, uint64_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint64_t r, carry;  // the carry out cannot be set here; it goes to an SGPR nobody reads
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(r), "=s"(carry) : "v"(a), "v"(b), "v"(c));
+  return r;
+#else
+  return (uint64_t)a * b + c;
+#endif
+}
+
+// 128-bit product of two 64-bit words: x0, x1 and the upper word V.
+struct GlWide {
+  uint32_t x0, x1;
+  uint64_t hi;
+};
+
+SC_HD GlWide gl_wide_mul(uint64_t a, uint64_t b) {
+  const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+  const uint64_t l = gl_mad(a0, b0, 0);
+  const uint64_t t = gl_mad((uint32_t)(l >> 32), 1u, gl_mad(a1, b0, 0));
+  const uint64_t u = gl_mad((uint32_t)t, 1u, gl_mad(a0, b1, 0));
+  const uint64_t v = gl_mad((uint32_t)(u >> 32), 1u, gl_mad((uint32_t)(t >> 32), 1u, gl_mad(a1, b1, 0)));
+  GlWide x;
+  x.x0 = (uint32_t)l;
+  x.x1 = (uint32_t)u;
+  x.hi = v;
+  return x;
+}
+
+// x 2^-64 mod p as a word below 2^64.
+SC_HD uint64_t gl_redc(const GlWide& x) {
+  uint32_t s1, b1, b2, b3, b4, b5, z;
+  const uint32_t s0 = gl_adc(x.x0, x.x1, 0u, s1);
+  uint32_t dh = gl_adc(s0, 0u, s1, z);
+  uint32_t dl = gl_sbb(0u, x.x1, 0u, b1);
+  dh = gl_sbb(dh, 0u, b1, z);
+  dl = gl_sbb(dl, 0u, s1, b2);
+  dh = gl_sbb(dh, 0u, b2, z);
+  uint32_t r0 = gl_sbb((uint32_t)x.hi, dl, 0u, b3);
+  uint32_t r1 = gl_sbb((uint32_t)(x.hi >> 32), dh, b3, b4);
+  r0 = gl_sbb(r0, 0u - b4, 0u, b5);
+  r1 = gl_sbb(r1, 0u, b5, z);
+  return (uint64_t)r0 | ((uint64_t)r1 << 32);
+}
+
+// x + d for any x < 2^64 and d < p: a carry out of bit 64 is worth 2^32 - 1, and adding it
+// cannot carry again (x + d < 2^65 - (2^32 - 1)).
+SC_HD uint64_t gl_add_fast(uint64_t x, uint64_t d) {
+  uint32_t c0, c1, c2, z;
+  uint32_t s0 = gl_adc((uint32_t)x, (uint32_t)d, 0u, c0);
+  uint32_t s1 = gl_adc((uint32_t)(x >> 32), (uint32_t)(d >> 32), c0, c1);
+  s0 = gl_adc(s0, 0u - c1, 0u, c2);
+  s1 = gl_adc(s1, 0u, c2, z);
+  return (uint64_t)s0 | ((uint64_t)s1 << 32);
+}
+
+// x - y for any x < 2^64 and y < p: a borrow is paid by adding p, i.e. subtracting 2^32 - 1
+// from the wrapped difference (which is then at least 2^64 - p). Canonical when x < p too.
+SC_HD uint64_t gl_sub_fast(uint64_t x, uint64_t y) {
+  uint32_t b0, b1, b2, z;
+  uint32_t d0 = gl_sbb((uint32_t)x, (uint32_t)y, 0u, b0);
+  uint32_t d1 = gl_sbb((uint32_t)(x >> 32), (uint32_t)(y >> 32), b0, b1);
+  d0 = gl_sbb(d0, 0u - b1, 0u, b2);
+  d1 = gl_sbb(d1, 0u, b2, z);
+  return (uint64_t)d0 | ((uint64_t)d1 << 32);
+}
+
+// x mod p for any x < 2^64: x + 2^32 - 1 carries out of bit 64 exactly when x >= p.
+SC_HD uint64_t gl_canon_fast(uint64_t x) {
+  uint32_t c0, c1;
+  const uint32_t t0 = gl_adc((uint32_t)x, 0xFFFFFFFFu, 0u, c0);
+  const uint32_t t1 = gl_adc((uint32_t)(x >> 32), 0u, c0, c1);
+  const uint32_t y0 = c1 ? t0 : (uint32_t)x, y1 = c1 ? t1 : (uint32_t)(x >> 32);
+  return (uint64_t)y0 | ((uint64_t)y1 << 32);
+}
+
+template <>
+struct Work<FieldP64> {
+  static constexpr int L = 2;
+  static constexpr bool kMontIsStored = true;
+  static constexpr bool kCheapMontInput = false;
+  static constexpr bool kLazyStep = false;
+  using V = Elem<2>;
+  SC_HD static V load(const uint32_t* w) {
+    V x;
+    x.v[0] = w[0];
+    x.v[1] = w[1];
+    return x;
+  }
+  SC_HD static V from_mont(const uint32_t* w) { return load(w); }
+  SC_HD static V from_stored(const uint32_t* w) { return load(w); }
+  SC_HD static V coeff(const uint32_t* w) { return load(w); }
+  SC_HD static void store(uint32_t* w, const V& x) {
+    w[0] = x.v[0];
+    w[1] = x.v[1];
+  }
+  SC_HD static V mul(const V& a, const V& b) { return gl_elem(gl_redc(gl_wide_mul(gl_u64(a), gl_u64(b)))); }
+  // a1 b1 + a2 b2: the reduced products summed lazily (the first made canonical)
+  SC_HD static V mul_sum(const V& a1, const V& b1, const V& a2, const V& b2) {
+    const uint64_t p1 = gl_canon_fast(gl_redc(gl_wide_mul(gl_u64(a1), gl_u64(b1))));
+    return gl_elem(gl_add_fast(gl_redc(gl_wide_mul(gl_u64(a2), gl_u64(b2))), p1));
+  }
+  // hi - lo for canonical hi, lo: canonical
+  SC_HD static V diff(const V& hi, const V& lo) { return gl_elem(gl_sub_fast(gl_u64(hi), gl_u64(lo))); }
+  // x + d for any x and canonical d
+  SC_HD static V step(const V& x, const V& d) { return gl_elem(gl_add_fast(gl_u64(x), gl_u64(d))); }
+  // lo + r (hi - lo) for canonical lo, hi and any r: canonical, so it can be stored and its
+  // differences stay canonical
+  SC_HD static V fold(const V& lo, const V& hi, const V& r) {
+    const uint64_t l = gl_u64(lo);
+    const uint64_t m = gl_redc(gl_wide_mul(gl_u64(r), gl_sub_fast(gl_u64(hi), l)));
+    return gl_elem(gl_canon_fast(gl_add_fast(m, l)));
+  }
+  // lo + t d for a per-lane 0 <= t < 8 (d canonical)
+  SC_HD static V at(const V& lo, const V& d, int t) {
+    uint64_t x = gl_u64(lo);
+    for (int s = 0; s < t; ++s) x = gl_add_fast(x, gl_u64(d));
+    return gl_elem(x);
+  }
+  // Running sum W = c0 + c1 2^32 + c2 2^64 + c3 2^96 of 32-bit words per 64-bit column, whose
+  // value is W 2^-64 mod p: a raw 128-bit product enters as its four words (one gl_mad each,
+  // no carries), a value x as x 2^64. A column grows by less than 2^32 per addition.
+  struct Acc {
+    uint64_t c0, c1, c2, c3;
+    SC_HD void clear() {
+      c0 = 0;
+      c1 = 0;
+      c2 = 0;
+      c3 = 0;
+    }
+    SC_HD void add_u64(uint64_t x) {
+      c2 = gl_mad((uint32_t)x, 1u, c2);
+      c3 = gl_mad((uint32_t)(x >> 32), 1u, c3);
+    }
+    SC_HD void add(const V& x) { add_u64(gl_u64(x)); }  // any x < 2^64
+    SC_HD void sub(const V& x) {                          // adds p - canon(x) (0 stays 0)
+      const uint64_t c = gl_canon_fast(gl_u64(x));
+      add_u64(c == 0 ? 0 : GL_P - c);
+    }
+    // W += a b for any a, b < 2^64
+    SC_HD void add_product(const V& a, const V& b) {
+      const GlWide x = gl_wide_mul(gl_u64(a), gl_u64(b));
+      c0 = gl_mad(x.x0, 1u, c0);
+      c1 = gl_mad(x.x1, 1u, c1);
+      c2 = gl_mad((uint32_t)x.hi, 1u, c2);
+      c3 = gl_mad((uint32_t)(x.hi >> 32), 1u, c3);
+    }
+    // Columns carried into W = w3:w2:w1:w0 + top 2^128 (top < 2^34), then
+    // W 2^-64 = redc(w) + top 2^64 with 2^64 = 2^32 - 1: top = t1 2^32 + t0 adds
+    // t0 (2^32 - 1) + t1 (2^64 - 2^32) = t0 (2^32 - 1) - t1 (mod p).
+    SC_HD Elem<2> value() const {
+      uint64_t s = (c0 >> 32) + (uint32_t)c1;
+      const uint32_t w0 = (uint32_t)c0, w1 = (uint32_t)s;
+      s = (s >> 32) + (c1 >> 32) + (uint32_t)c2;
+      const uint32_t w2 = (uint32_t)s;
+      s = (s >> 32) + (c2 >> 32) + (uint32_t)c3;
+      const uint32_t w3 = (uint32_t)s;
+      const uint64_t top = (s >> 32) + (c3 >> 32);
+      GlWide x;
+      x.x0 = w0;
+      x.x1 = w1;
+      x.hi = (uint64_t)w2 | ((uint64_t)w3 << 32);
+      uint64_t r = gl_canon(gl_redc(x));
+      r = gl_add(r, gl_canon((uint64_t)(uint32_t)top * GL_EPS));
+      return gl_elem(gl_sub(r, top >> 32));
+    }
+  };
+};
+
 // Host: Montgomery coefficient -> the word form Work<F>::coeff reads.
 template <class F>
 inline Elem<F::L> work_coeff(const Elem<F::L>& c) {

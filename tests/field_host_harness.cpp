@@ -186,3 +186,31 @@ extern "C" uint64_t gl_host_accum(uint64_t n, const int32_t* ops, const uint64_t
   }
   return gl_u64(acc.value());
 }
+
+// The same operations on the Goldilocks working form's running sum (Work<FieldP64>::Acc:
+// 64-bit column sums of 32-bit words, carried once in value()).
+extern "C" uint64_t gl_host_work_accum(uint64_t n, const int32_t* ops, const uint64_t* a, const uint64_t* b) {
+  Work<FieldP64>::Acc acc;
+  acc.clear();
+  for (uint64_t i = 0; i < n; ++i) {
+    if (ops[i] == 0) acc.add(gl_elem(a[i]));
+    else if (ops[i] == 1) acc.sub(gl_elem(a[i]));
+    else acc.add_product(gl_elem(a[i]), gl_elem(b[i]));
+  }
+  return gl_u64(acc.value());
+}
+
+// Goldilocks working-form primitives on raw 64-bit words: op 0 gl_redc(gl_wide_mul(a, b)),
+// op 1 gl_add_fast(a, b), op 2 gl_sub_fast(a, b), op 3 gl_canon_fast(a); op 4 the high word
+// of gl_wide_mul(a, b) and op 5 its low word.
+extern "C" void gl_host_fast(int op, uint64_t n, const uint64_t* a, const uint64_t* b, uint64_t* out) {
+  for (uint64_t i = 0; i < n; ++i) {
+    const GlWide x = gl_wide_mul(a[i], b[i]);
+    if (op == 0) out[i] = gl_redc(x);
+    else if (op == 1) out[i] = gl_add_fast(a[i], b[i]);
+    else if (op == 2) out[i] = gl_sub_fast(a[i], b[i]);
+    else if (op == 3) out[i] = gl_canon_fast(a[i]);
+    else if (op == 4) out[i] = x.hi;
+    else out[i] = (uint64_t)x.x0 | ((uint64_t)x.x1 << 32);
+  }
+}

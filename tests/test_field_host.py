@@ -39,6 +39,10 @@ def harness(tmp_path_factory):
     lib.interleave_host.restype = None
     lib.gl_host_accum.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32), u64p, u64p]
     lib.gl_host_accum.restype = ctypes.c_uint64
+    lib.gl_host_work_accum.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32), u64p, u64p]
+    lib.gl_host_work_accum.restype = ctypes.c_uint64
+    lib.gl_host_fast.argtypes = [ctypes.c_int, ctypes.c_uint64, u64p, u64p, u64p]
+    lib.gl_host_fast.restype = None
 
     def op(bw, code, a, b):
         f = get_field(bw)
@@ -77,11 +81,20 @@ def harness(tmp_path_factory):
                             back.ctypes.data_as(u64p))
         return e, o, back
 
-    def gl_accum(ops, a, b):
+    def gl_accum(ops, a, b, work_form=False):
         o = np.ascontiguousarray(ops, dtype=np.int32)
         A, B = (np.ascontiguousarray(x, dtype=np.uint64) for x in (a, b))
-        return int(lib.gl_host_accum(len(o), o.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                                     A.ctypes.data_as(u64p), B.ctypes.data_as(u64p)))
+        fn = lib.gl_host_work_accum if work_form else lib.gl_host_accum
+        return int(fn(len(o), o.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                      A.ctypes.data_as(u64p), B.ctypes.data_as(u64p)))
+
+    def gl_fast(code, a, b):
+        A, B = (np.ascontiguousarray(x, dtype=np.uint64) for x in (a, b))
+        out = np.zeros_like(A)
+        lib.gl_host_fast(code, len(A), A.ctypes.data_as(u64p), B.ctypes.data_as(u64p), out.ctypes.data_as(u64p))
+        return [int(v) for v in out]
+
+    op.gl_fast = gl_fast
 
     op.gl_mul_sum = gl_mul_sum
     op.gl_accum = gl_accum
@@ -227,9 +240,11 @@ def test_host_keccak_bit_interleaving(harness):
         assert re.tolist() == want_e and ro.tolist() == want_o, r
 
 
-def test_host_goldilocks_wide_accumulator(harness):
-    """The Goldilocks running sum holds W = top 2^128 + w_hi 2^64 + w_lo with value W R^-1:
-    values enter as X 2^64, products A B raw, and one reduction at the end. Checked against
+@pytest.mark.parametrize("work_form", [False, True], ids=["accum", "work_columns"])
+def test_host_goldilocks_wide_accumulator(harness, work_form):
+    """The Goldilocks running sums hold an integer W whose value is W R^-1: values enter as
+    X 2^64, products A B raw, and one reduction at the end (Accum: W = top 2^128 + w_hi 2^64 +
+    w_lo; the working form's Acc: four 64-bit column sums of 32-bit words). Checked against
     Python integers on long runs of near-2^64 operands, where every carry out of bit 63 and
     of bit 127 happens many times."""
     p = (1 << 64) - (1 << 32) + 1
@@ -252,4 +267,38 @@ def test_host_goldilocks_wide_accumulator(harness):
                 want -= x
             else:
                 want += x * y * rinv
-        assert harness.gl_accum(ops, a, b) == want % p, case
+        assert harness.gl_accum(ops, a, b, work_form) == want % p, case
+
+
+def test_host_goldilocks_working_primitives(harness):
+    """The Goldilocks working form's carry-free product and its reduction, on raw 64-bit
+    words (the kernels pass products and lazy sums anywhere below 2^64): the 128-bit product
+    is exact, x 2^-64 is congruent and below 2^64; x + d and x - d for any x and d < p are
+    congruent and below 2^64 (the difference canonical when x < p); canon is x mod p."""
+    p = (1 << 64) - (1 << 32) + 1
+    rinv = pow(1 << 64, -1, p)
+    rng = random.Random(6464)
+    M = (1 << 64) - 1
+    edge = [0, 1, 2, (1 << 32) - 1, 1 << 32, (1 << 32) + 1, p - 2, p - 1, p, p + 1, M - 1, M,
+            1 << 63, (1 << 63) - 1, 0xFFFFFFFF00000000, 0x00000000FFFFFFFF]
+    a = [x for x in edge for _ in edge]
+    b = [y for _ in edge for y in edge]
+    for _ in range(30000):
+        a.append(rng.choice((rng.getrandbits(64), M - rng.randrange(1 << 24), rng.randrange(p),
+                             rng.randrange(1 << 32) << 32, rng.randrange(1 << 33))))
+        b.append(rng.choice((rng.getrandbits(64), M - rng.randrange(1 << 24), rng.randrange(p))))
+    lo = harness.gl_fast(5, a, b)
+    hi = harness.gl_fast(4, a, b)
+    red = harness.gl_fast(0, a, b)
+    for x, y, l, h, r in zip(a, b, lo, hi, red):
+        assert (h << 64) | l == x * y, (x, y)
+        assert r <= M and r % p == x * y * rinv % p, (x, y)
+    bc = [y % p for y in b]  # d and y canonical
+    for x, d, s in zip(a, bc, harness.gl_fast(1, a, bc)):
+        assert s <= M and s % p == (x + d) % p, (x, d)
+    for x, d, s in zip(a, bc, harness.gl_fast(2, a, bc)):
+        assert s <= M and s % p == (x - d) % p, (x, d)
+        if x < p:
+            assert s < p, (x, d)
+    for x, c in zip(a, harness.gl_fast(3, a, b)):
+        assert c == x % p, x
