@@ -58,13 +58,18 @@ def pass_bytes(T: int, limbs: int, n0: int, j: int) -> int:
 
 
 def modmuls_per_pair(spec, fold: bool) -> int:
-    """Field multiplications per pair the kernel performs in a round: the composition at each
-    evaluated point (each term's len-1 products, plus one for a coefficient other than +-1;
-    d+1 points in round 0, d from round 1 on, where g(1) comes from the claim) and, when the
-    round folds, two per table."""
-    per_point = sum(n - 1 for n in spec.term_lens) + sum(1 for c in spec.coeffs if c not in (1, -1))
-    points = spec.degree if fold else spec.degree + 1
-    return points * per_point + (2 * spec.num_vars if fold else 0)
+    """Field multiplications per pair the kernel's algorithm needs in a round: the composition
+    at each evaluated point (each term's len-1 products, plus one for a coefficient other than
+    +-1) at t = 0..d-1 (t = 1 only in round 0: later rounds take g(1) from the claim), and the
+    leading coefficient, which only the degree-d terms enter (the kernels evaluate t = d at
+    infinity when d >= 2); when the round folds, two per table."""
+    def products(terms):
+        return sum(n - 1 + (1 if c not in (1, -1) else 0) for c, n in terms)
+    terms = list(zip(spec.coeffs, spec.term_lens))
+    d = spec.degree
+    finite = (d - 1 if fold else d) if d >= 2 else (d if fold else d + 1)
+    lead = products([(c, n) for c, n in terms if n == d]) if d >= 2 else 0
+    return finite * products(terms) + lead + (2 * spec.num_vars if fold else 0)
 
 
 def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: float | None,

@@ -109,9 +109,11 @@ __device__ __forceinline__ Elem<L> load_elem(const uint32_t* p) {
 
 // C(x) = sum_k c_k prod_v x_v^e_kv. Term structure is wave-uniform, so every branch below is
 // a scalar branch; table slots are indexed statically (unrolled v), keeping x in VGPRs.
+// Terms with fewer than min_len factors are left out (the point-parallel lane that evaluates
+// the leading coefficient passes the degree).
 template <class F, int TMAX, bool EXPAND = (TMAX <= 8)>
 __device__ __forceinline__ void compose_add(typename Work<F>::Acc& acc, const typename Work<F>::V (&x)[TMAX],
-                                            const CompDesc* __restrict__ cd, int K) {
+                                            const CompDesc* __restrict__ cd, int K, int min_len = 0) {
   using W = Work<F>;
   for (int k = 0; k < K; ++k) {
     typename W::V prod = x[0];  // every term has a factor, so prod is set before it is used
@@ -129,12 +131,13 @@ __device__ __forceinline__ void compose_add(typename Work<F>::Acc& acc, const ty
       }
     });
     const int kind = cd->kind[k];
-    if (kind == COEFF_ONE) {
-      acc.add(prod);
-    } else if (kind == COEFF_MINUS_ONE) {
-      acc.sub(prod);
-    } else {
-      acc.add(W::mul(prod, W::coeff(cd->coeff[k])));
+    if (kind == COEFF_GENERAL) prod = W::mul(prod, W::coeff(cd->coeff[k]));
+    if (cd->len[k] >= min_len) {
+      if (kind == COEFF_MINUS_ONE) {
+        acc.sub(prod);
+      } else {
+        acc.add(prod);
+      }
     }
   }
 }
@@ -211,6 +214,75 @@ __device__ __forceinline__ void compose(typename Work<F>::Acc& acc, const typena
   }
 }
 
+// The leading coefficient of a pair's polynomial C(lo + t (hi - lo)) (degree d): the terms of
+// d factors, each factor replaced by its slope s_v = hi_v - lo_v. acc += that coefficient.
+template <class F, int TMAX, bool EXPAND = (TMAX <= 8)>
+__device__ __forceinline__ void compose_add_lead(typename Work<F>::Acc& acc, const typename Work<F>::V (&s)[TMAX],
+                                                 const CompDesc* __restrict__ cd, int K, int d) {
+  using W = Work<F>;
+  for (int k = 0; k < K; ++k) {
+    if (cd->len[k] != d) continue;  // wave-uniform
+    typename W::V prod = s[0];
+    bool have = false;
+    table_for<TMAX, EXPAND>([&](auto vi) {
+      const int v = vi;
+      uint32_t e = (cd->exps[k][v >> 3] >> ((v & 7) * 4)) & 15u;
+      for (; e != 0u; --e) {
+        if (have) {
+          prod = W::mul(prod, s[v]);
+        } else {
+          prod = s[v];
+          have = true;
+        }
+      }
+    });
+    const int kind = cd->kind[k];
+    if (kind == COEFF_ONE) {
+      acc.add(prod);
+    } else if (kind == COEFF_MINUS_ONE) {
+      acc.sub(prod);
+    } else {
+      acc.add(W::mul(prod, W::coeff(cd->coeff[k])));
+    }
+  }
+}
+
+template <class F, int SID, int TMAX>
+__device__ __forceinline__ void compose_static_lead(typename Work<F>::Acc& acc, const typename Work<F>::V (&s)[TMAX],
+                                                    const CompDesc* __restrict__ cd) {
+  using W = Work<F>;
+  constexpr int D = static_degree(SID);
+  static_for<STATIC_COMPS[SID].num_terms>([&](auto ki) {
+    constexpr int k = decltype(ki)::value;
+    constexpr StaticComp C = STATIC_COMPS[SID];
+    if constexpr (C.lens[k] == D) {
+      typename W::V prod = s[C.vars[k][0]];
+      constexpr int kept = C.kinds[k] == COEFF_ONE ? 1 : 0;
+      static_for<C.lens[k] - 1 - kept>([&](auto mi) {
+        constexpr int m = decltype(mi)::value + 1;
+        prod = W::mul(prod, s[STATIC_COMPS[SID].vars[k][m]]);
+      });
+      if constexpr (kept == 1) {
+        acc_add_mul<F>(acc, prod, s[C.vars[k][C.lens[k] - 1]]);
+      } else if constexpr (C.kinds[k] == COEFF_MINUS_ONE) {
+        acc.sub(prod);
+      } else {
+        acc_add_mul<F>(acc, prod, W::coeff(cd->coeff[k]));
+      }
+    }
+  });
+}
+
+template <class F, int SID, int TMAX>
+__device__ __forceinline__ void compose_lead(typename Work<F>::Acc& acc, const typename Work<F>::V (&s)[TMAX],
+                                             const CompDesc* __restrict__ cd, int K, int d) {
+  if constexpr (SID == 0) {
+    compose_add_lead<F, TMAX>(acc, s, cd, K, d);
+  } else {
+    compose_static_lead<F, SID, TMAX>(acc, s, cd);
+  }
+}
+
 // Shape of the composition a kernel runs: compile-time for compiled ones.
 template <int SID>
 struct Shape {
@@ -226,15 +298,18 @@ struct Shape {
 };
 
 // cur holds f[2i] and oth holds f[2i+1] of every table on entry. Accumulates
-// C(f[2i] + t (f[2i+1] - f[2i])) into acc[t] for t < npts, with t >= 2 reached by repeated
-// addition of the difference (reference.py:89-99); those sums may stay unreduced where the
-// field allows (step_add).
-template <class F, int TMAX, int PMAX, int SID>
+// C(f[2i] + t (f[2i+1] - f[2i])) into acc[t] for t < d = npts - 1, with t >= 2 reached by
+// repeated addition of the difference (reference.py:89-99; those sums may stay unreduced
+// where the field allows, step_add), and into acc[d] either the value at t = d (LEAD false)
+// or, for d >= 2, the polynomial's leading coefficient (its value at infinity: the degree-d
+// terms over the differences), from which the round's finish recovers g(d).
+template <class F, int TMAX, int PMAX, int SID, bool LEAD = true>
 __device__ __forceinline__ void eval_points(typename Work<F>::V (&cur)[TMAX], typename Work<F>::V (&oth)[TMAX],
                                             typename Work<F>::Acc (&acc)[PMAX], const CompDesc* __restrict__ cd,
                                             int T, int K, int npts, bool skip1) {
   using W = Work<F>;
   constexpr bool X = SID != 0 || TMAX <= 8;  // expand every loop unless the table count is large
+  const int d = LEAD ? npts - 1 : npts;  // the points t < d are evaluated
   compose<F, SID, TMAX>(acc[0], cur, cd, K);
   table_for<TMAX, X>([&](auto vi) {
     const int v = vi;
@@ -247,7 +322,8 @@ __device__ __forceinline__ void eval_points(typename Work<F>::V (&cur)[TMAX], ty
   if (!skip1) compose<F, SID, TMAX>(acc[1], cur, cd, K);  // else finish_round derives g(1)
   table_for<PMAX - 2, X>([&](auto ti) {
     const int t = (int)ti + 2;
-    if (t < npts) {
+    if (t == d) compose_lead<F, SID, TMAX>(acc[t], oth, cd, K, d);
+    if (t < d) {
       table_for<TMAX, X>([&](auto vi) {
         const int v = vi;
         if constexpr (SID != 0 && W::kLazyStep) {
@@ -521,23 +597,59 @@ __device__ __forceinline__ Elem<F::L> previous_claim(const ProofState* st, int r
   return group8_sum<F>(term);
 }
 
-// With fill_g1 (every prover path; the per-pair kernels skip point 1 from round 1 on),
-// g(1) of round >= 1 is set to claim - g(0): the protocol's own identity
-// g_j(0) + g_j(1) = g_{j-1}(r_{j-1}), so the value is the one evaluation would give.
+// g(d) of a degree-d >= 2 polynomial from g(0..d-1) and its leading coefficient in g[d]
+// (FieldConsts::lead), one term per lane of the calling wave; the value goes to lanes 0..7.
+template <class F>
+__device__ __forceinline__ Elem<F::L> value_from_lead(const uint32_t* g, int npts, const FieldConsts* __restrict__ fc) {
+  using W = Work<F>;
+  constexpr int L = F::L;
+  const int t = threadIdx.x & 63;
+  const int d = npts - 1;
+  Elem<L> term = zero<F>();
+  if (t <= d) {
+    typename W::Acc acc;
+    acc.clear();
+    acc.add(W::mul(W::from_mont(g + t * L), W::from_mont(fc->lead[d][t])));
+    term = acc.value();
+  }
+  return group8_sum<F>(term);
+}
+
+// With from_sums (every prover path), g holds the point sums of the round's kernels: from
+// round 1 on g(1) is missing (the per-pair and point-parallel kernels skip point 1) and is
+// set to claim - g(0), the protocol's own identity g_j(0) + g_j(1) = g_{j-1}(r_{j-1}). With
+// lead as well (the pass, point-parallel and batched stage kernels; not the tail kernel), for
+// a degree d >= 2 g[d] holds the sum of the leading coefficients (the kernels evaluate the
+// pairs' polynomials at infinity instead of at t = d), from which g(d) follows by the finite
+// difference identity. Both values are the ones evaluation at the point would give.
 template <class F, int NT>
 __device__ __forceinline__ void finish_round(uint32_t* g, int npts, int round, int num_rounds, int mode,
                              const FieldConsts* __restrict__ fc, ProofState* st,
                              const uint32_t* __restrict__ explicit_r, uint32_t* msg, uint32_t* r_out,
-                             bool last_challenge = false, bool fill_g1 = false) {
+                             bool last_challenge = false, bool from_sums = false, bool lead = false) {
   constexpr int L = F::L;
   const int tid = threadIdx.x;
-  if (fill_g1 && round > 0) {
+  const bool fill1 = from_sums && round > 0, filld = from_sums && lead && npts >= 3;
+  if (fill1 || filld) {
+    // one wave; g(1) is written before the lanes read g for g(d)
     if (tid < 64) {
-      const Elem<L> claim = previous_claim<F>(st, round, npts, fc);
-      if (tid == 0) {
-        const Elem<L> g1 = sub<F>(claim, load_elem<L>(g));
+      if (fill1) {
+        const Elem<L> claim = previous_claim<F>(st, round, npts, fc);
+        if (tid == 0) {
+          const Elem<L> g1 = sub<F>(claim, load_elem<L>(g));
 #pragma unroll
-        for (int k = 0; k < L; ++k) g[L + k] = g1.v[k];
+          for (int k = 0; k < L; ++k) g[L + k] = g1.v[k];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+      if (filld) {
+        const Elem<L> gd = value_from_lead<F>(g, npts, fc);
+        if (tid == 0) {
+#pragma unroll
+          for (int k = 0; k < L; ++k) g[(npts - 1) * L + k] = gd.v[k];
+        }
       }
     }
     __syncthreads();
@@ -612,7 +724,7 @@ __device__ __forceinline__ void reduce_partials_and_finish(const uint32_t* parti
     }
     __syncthreads();
   }
-  finish_round<F, BLOCK>(s_g, npts, round, num_rounds, mode, fc, st, explicit_r, s_msg, s_r, false, true);
+  finish_round<F, BLOCK>(s_g, npts, round, num_rounds, mode, fc, st, explicit_r, s_msg, s_r, false, true, true);
 }
 
 template <class F, int TMAX, int PMAX, int SID>
@@ -771,14 +883,14 @@ __device__ __forceinline__ void tail_rounds(const TailArgs& a) {
     for (uint64_t i = threadIdx.x; i < pairs; i += TB) {
       typename W::V cur[TMAX], oth[TMAX];
       load_pair<F, TMAX>(cur, oth, src, dst, src_n, src_st, i, T, fold, j <= 1, r);
-      eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts, fold);
+      eval_points<F, TMAX, PMAX, SID, false>(cur, oth, acc, cd, T, K, npts, fold);
     }
     Elem<L> sums[PMAX];
 #pragma unroll
     for (int t = 0; t < PMAX; ++t) sums[t] = acc[t].value();
     block_reduce<F, PMAX, TB>(sums, npts, s_red, s_g);
     finish_round<F, TB>(s_g, npts, j, a.num_rounds, a.challenge_mode, a.fc, a.st, a.explicit_r, s_msg, s_r,
-                        a.want_final != 0, true);
+                        a.want_final != 0, true, false);
     __syncthreads();
   }
   if (!a.want_final) return;
@@ -889,7 +1001,7 @@ __device__ __forceinline__ typename Work<F>::V pick_slot(const typename Work<F>:
 // starts from its first factor.
 template <class F, int N>
 __device__ __forceinline__ void compose_slots(typename Work<F>::Acc& acc, const typename Work<F>::V (&x)[N],
-                                              const CompDesc* __restrict__ cd, int K) {
+                                              const CompDesc* __restrict__ cd, int K, int min_len) {
   using W = Work<F>;
   for (int k = 0; k < K; ++k) {
     const int len = cd->len[k];
@@ -903,7 +1015,9 @@ __device__ __forceinline__ void compose_slots(typename Work<F>::Acc& acc, const 
       m = 1;
     }
     for (; m < len; ++m) mul_slot<F, N>(prod, x, cd->fac[k][m]);
-    if (kind == COEFF_MINUS_ONE) acc.sub(prod); else acc.add(prod);
+    if (len >= min_len) {
+      if (kind == COEFF_MINUS_ONE) acc.sub(prod); else acc.add(prod);
+    }
   }
 }
 
@@ -927,6 +1041,10 @@ __device__ __forceinline__ void wide_eval_block(const uint32_t* __restrict__ src
   const int q = threadIdx.x % P;
   const int group = threadIdx.x / P;
   const int t = (round > 0 && q >= 1) ? q + 1 : q;
+  // the lane of t = d >= 2 evaluates the leading coefficient (the round's finish turns it
+  // into g(d)): the differences as the values, terms of d factors only
+  const bool lead = npts >= 3 && t == npts - 1;
+  const int min_len = lead ? npts - 1 : 0;
   typename W::Acc acc;
   acc.clear();
   if (group < pairs_here) {
@@ -939,7 +1057,8 @@ __device__ __forceinline__ void wide_eval_block(const uint32_t* __restrict__ src
           load_words<2 * L>(w, src + ((uint64_t)v * src_st + 2 * i) * L);
           const typename W::V lo = src_mont ? W::from_mont(w) : W::from_stored(w);
           const typename W::V hi = src_mont ? W::from_mont(w + L) : W::from_stored(w + L);
-          x[v] = W::at(lo, W::diff(hi, lo), t);
+          const typename W::V dv = W::diff(hi, lo);
+          x[v] = lead ? dv : W::at(lo, dv, t);
         } else {
           x[v] = x[0];  // defined (an undefined slot measured 50 more VGPRs), never read
         }
@@ -948,9 +1067,9 @@ __device__ __forceinline__ void wide_eval_block(const uint32_t* __restrict__ src
       // and keep the term loop (x then lives in scratch memory).
       constexpr bool kSlots = TMAX > 8 && TMAX * (int)(sizeof(typename W::V) / 4) <= 160;
       if constexpr (kSlots) {
-        compose_slots<F, TMAX>(acc, x, cd, K);
+        compose_slots<F, TMAX>(acc, x, cd, K, min_len);
       } else {
-        compose_add<F, TMAX>(acc, x, cd, K);
+        compose_add<F, TMAX>(acc, x, cd, K, min_len);
       }
     }
   }
@@ -1136,7 +1255,7 @@ __global__ void __launch_bounds__(BLOCK) batch_finalize_kernel(BatchArgs a) {
   }
   block_reduce<F, PMAX, BLOCK>(acc, npts, s_red, s_g);
   finish_round<F, BLOCK>(s_g, npts, (int)ent.round, in.num_vars, in.challenge_mode, a.fc, in.st, nullptr,
-                         s_msg, s_r, false, true);
+                         s_msg, s_r, false, true, true);
 }
 
 // Input generation and Montgomery encoding (cuda/src/field_sumcheck.cu:593-611).

@@ -133,6 +133,10 @@ struct FieldConsts {
   // g(r) = sum_t g(t) bary[n-1][t] prod_{s != t} (r - s) for a polynomial known at 0..n-1.
   uint32_t bary[MAX_POINTS][MAX_POINTS][MAX_LIMBS];
   uint32_t pow32[MAX_POINTS][MAX_LIMBS];  // Montgomery images of 32^k, k < 8
+  // g(d) of a degree-d polynomial from g(0..d-1) and its leading coefficient a (d >= 2):
+  // sum_{t <= d} (-1)^(d-t) C(d,t) g(t) = d! a, so g(d) = d! a + sum_{t<d} lead[d][t] g(t) with
+  // lead[d][t] = (-1)^(d-t+1) C(d,t) and lead[d][d] = d!, Montgomery.
+  uint32_t lead[MAX_POINTS][MAX_POINTS][MAX_LIMBS];
 };
 
 // Per-proof device state. polys and r are written by the kernel that finishes a round.

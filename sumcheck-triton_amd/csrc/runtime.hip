@@ -116,6 +116,19 @@ struct HostField {
       for (int i = 0; i < L; ++i) fc.pow32[k][i] = pw.v[i];
       pw = dbl_times(pw, 5);
     }
+    for (int d = 2; d < MAX_POINTS; ++d) {
+      int64_t binom = 1;  // C(d, t)
+      for (int t = 0; t <= d; ++t) {
+        Elem<L> w = fact[d];  // t = d: d!
+        if (t < d) {
+          w = zero<F>();
+          for (int64_t k = 0; k < binom; ++k) w = add<F>(w, rmod);
+          if (((d - t + 1) & 1) != 0) w = sub<F>(zero<F>(), w);
+        }
+        for (int i = 0; i < L; ++i) fc.lead[d][t][i] = w.v[i];
+        binom = binom * (d - t) / (t + 1);
+      }
+    }
   }
   // x^(p-2) in the Montgomery domain (the Montgomery image of x^-1 for x != 0).
   static Elem<L> mont_inverse(const Elem<L>& x, const Elem<L>& one_mont) {
