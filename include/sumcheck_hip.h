@@ -103,6 +103,16 @@ int sc_proof_set_schedule(sc_proof* proof, int32_t max_pass_blocks, int64_t tail
  * fold kernel (field_sumcheck.cu:614-694). */
 int sc_proof_set_point_parallel(sc_proof* proof, int64_t max_pairs);
 
+/* Layout of the intermediate (folded) tables: chunks of 2^chunk_log2 entries of every table
+ * side by side (chunk q of table v after chunk q of tables 0..v-1), so a wave's pairs of all
+ * tables form one contiguous run; a chunk at least a table's size makes them table-major.
+ * chunk_log2 in [2, 63] (a folding round reads 4 consecutive entries of each table); 0 restores
+ * the default (2^7 up to 128 bits, table-major at 256 bits).
+ * Results do not depend on it. No
+ * counterpart in the reference, whose tables are table-major throughout
+ * (field_sumcheck.cu:130-190). */
+int sc_proof_set_fold_chunk(sc_proof* proof, int32_t chunk_log2);
+
 /* Enqueues one whole proof (all rounds, transcript on the device) on the context's stream
  * and returns without waiting: the reference's run_once (field_sumcheck.cu:1012-1088).
  * explicit_r: [num_vars-1][L] canonical, only for SC_CHALLENGE_EXPLICIT (else NULL). */

@@ -158,6 +158,11 @@ struct ProofState {
 // 5.0-5.5 TB/s against 6.1-6.2 TB/s with a 12 KiB gap after each table (a 128 MiB table is
 // not helped, so smaller tables stay packed). Fold buffers are written and read in the 2:1
 // read/write mix, which the gap does not change, and stay packed.
+// Fold buffers' chunk (kernels.hpp, TableMap), log2 of its entries: 2^7 (a wave's 64 pairs of
+// round j + 1 read two chunks of each table) for fields of up to 4 limbs, table-major for the
+// 255-bit field. Same-box timings on MI355X, a*b*c + d*e pass kernels, chunked vs table-major:
+// +1-2% at 32, 64 and 128 bits, -1.5% at 256 bits (whose folding rounds are issue-bound).
+constexpr int32_t default_fold_chunk_shift(int limbs) { return limbs == 8 ? 63 : 7; }
 constexpr uint64_t INPUT_PAD_BYTES = 12u << 10;
 constexpr uint64_t INPUT_PAD_MIN_TABLE_BYTES = (uint64_t)1 << 29;
 constexpr uint64_t input_stride(uint64_t n, int limbs) {
@@ -175,7 +180,7 @@ struct PassArgs {
   int32_t fold;  // 0 for round 0 (pairs read straight from src), 1 otherwise
   int32_t challenge_mode;
   int32_t fused_finalize;  // the last block to finish reduces the partials and derives r
-  int32_t pad0;
+  int32_t chunk_shift;     // fold buffers' chunk, log2 of its entries (63: table-major)
   const CompDesc* comp;
   const FieldConsts* fc;
   ProofState* st;
@@ -195,6 +200,8 @@ struct TailArgs {
   int32_t num_rounds;
   int32_t challenge_mode;
   int32_t want_final;  // also derive the last challenge and fold the final point
+  int32_t chunk_shift;  // fold buffers' chunk, log2 of its entries (63: table-major)
+  int32_t pad0;
   const CompDesc* comp;
   const FieldConsts* fc;
   ProofState* st;
@@ -256,7 +263,7 @@ struct BatchArgs {
   const InstDesc* insts;
   const StageEntry* entries;  // sorted by tile_begin
   uint32_t n_entries;
-  uint32_t pad1;
+  int32_t chunk_shift;        // fold buffers' chunk, log2 of its entries (63: table-major)
   uint32_t* partials;         // [tiles][MAX_POINTS][MAX_LIMBS]
   const FieldConsts* fc;
   uint32_t tile_base;         // point-parallel evaluation: block b is tile b + tile_base

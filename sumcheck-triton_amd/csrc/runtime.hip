@@ -281,6 +281,7 @@ struct sc_proof {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int max_pass_blocks = 2048;
   int64_t tail_pairs = 0;
+  int32_t fold_chunk_shift = 7;  // fold buffers' chunk (log2 entries; init_shape sets the default)
   int64_t wide_pairs = 0;      // rounds with at most this many pairs run point-parallel
   bool default_wide = true;    // wide_pairs follows the kernel choice (no caller-set value)
   bool fused_finalize = false;
@@ -478,7 +479,7 @@ int enqueue_rounds(sc_proof* p, int mode) {
     a.partials = p->d_partials;
     a.explicit_r = p->d_explicit;
     a.fused_finalize = p->fused_finalize ? 1 : 0;
-    a.pad0 = 0;
+    a.chunk_shift = p->fold_chunk_shift;
     uint64_t g = (a.pairs + BLOCK - 1) / BLOCK;
     if (g > (uint64_t)p->max_pass_blocks) g = (uint64_t)p->max_pass_blocks;
     const bool wide = (int64_t)a.pairs <= p->wide_pairs;
@@ -528,6 +529,8 @@ int enqueue_rounds(sc_proof* p, int mode) {
     t.buf_b = p->d_b;
     t.n0 = p->n0;
     t.in_stride = p->in_stride;
+    t.chunk_shift = p->fold_chunk_shift;
+    t.pad0 = 0;
     t.first_round = jt;
     t.num_rounds = p->num_vars;
     t.challenge_mode = mode;
@@ -585,6 +588,7 @@ void init_shape(sc_proof* p, sc_context* ctx, int fidx, const CompDesc& desc, in
   p->n0 = (uint64_t)1 << num_vars;
   p->tail_pairs = default_proof_tail_pairs(p->L, p->tmax > 8);
   p->max_pass_blocks = default_pass_blocks(p->L);
+  p->fold_chunk_shift = default_fold_chunk_shift(p->L);
   p->matched_sid = match_static(desc);
   p->sid = p->matched_sid;
   p->wide_pairs = default_wide_pairs(p);
@@ -863,6 +867,16 @@ int sc_proof_set_point_parallel(sc_proof* p, int64_t max_pairs) {
   if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
   p->default_wide = max_pairs < 0;
   p->wide_pairs = max_pairs < 0 ? default_wide_pairs(p) : max_pairs;
+  drop_graph(p);
+  return SC_OK;
+}
+
+int sc_proof_set_fold_chunk(sc_proof* p, int32_t chunk_log2) {
+  if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
+  // a folding round reads 4 consecutive entries of a table, so a chunk holds at least 4
+  if (chunk_log2 < 0 || chunk_log2 == 1 || chunk_log2 > 63)
+    return fail(SC_ERR_ARG, "chunk_log2 must be 0 (default) or in [2, 63]");
+  p->fold_chunk_shift = chunk_log2 == 0 ? default_fold_chunk_shift(p->L) : chunk_log2;
   drop_graph(p);
   return SC_OK;
 }
@@ -1330,7 +1344,7 @@ int enqueue_batch(sc_batch* b) {
   for (int st = 0; st < b->num_stages; ++st) {
     BatchArgs a;
     a.insts = b->d_insts;
-    a.pad1 = 0;
+    a.chunk_shift = default_fold_chunk_shift(b->L);
     a.partials = b->d_partials;
     a.fc = b->ctx->d_fc[b->fidx];
     a.tile_base = 0;
@@ -2099,7 +2113,7 @@ int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
     a.tile_base = 0;
     a.pad0 = 0;
     a.insts = sv->d_insts;
-    a.pad1 = 0;
+    a.chunk_shift = default_fold_chunk_shift(launchers_for(fidx).limbs);
     a.partials = sv->d_partials;
     a.fc = ctx->d_fc[fidx];
     const Launchers& ln = launchers_for(fidx);

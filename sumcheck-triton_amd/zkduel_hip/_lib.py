@@ -72,6 +72,7 @@ SIGNATURES = [
       ctypes.POINTER(ScInput), ctypes.POINTER(ctypes.c_void_p)]),
     ("sc_proof_set_schedule", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32]),
     ("sc_proof_set_point_parallel", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    ("sc_proof_set_fold_chunk", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("sc_proof_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, u32p]),
     ("sc_proof_capture", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, u32p]),
     ("sc_proof_wait", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),

@@ -238,6 +238,12 @@ class Proof:
         instead of the per-pair pass kernel (0: never; negative: the default rule)."""
         _lib.check(self.lib.sc_proof_set_point_parallel(self.handle, int(max_pairs)))
 
+    def set_fold_chunk(self, chunk_log2: int = 0) -> None:
+        """Intermediate tables in chunks of 2^chunk_log2 entries (chunk_log2 >= 2) of every table side by side
+        (0: the default, 2^7 up to 128 bits and table-major at 256 bits; a chunk at least a
+        table's size: table-major). Results do not depend on it."""
+        _lib.check(self.lib.sc_proof_set_fold_chunk(self.handle, int(chunk_log2)))
+
     def run(self, challenge_mode: str = "sha3", challenges=None) -> None:
         mode = _lib.CHALLENGE_MODES[challenge_mode]
         r = self._explicit(mode, challenges)

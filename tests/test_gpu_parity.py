@@ -352,6 +352,7 @@ def test_schedule_and_graph_replays_are_identical(ctx, bw):
         pr.run("sha3")
         pr.wait()
         base = pr.result()
+        base_final = pr.final()
         for blocks, tail, fused in ((16, 1, False), (4096, 8, False), (2048, 1 << 20, False),
                                     (16, 1, True), (4096, 8, True), (0, 0, True), (0, 0, False)):
             say("schedule", bw, blocks, tail, fused)
@@ -360,6 +361,25 @@ def test_schedule_and_graph_replays_are_identical(ctx, bw):
             pr.wait()
             got = pr.result()
             assert np.array_equal(got[0], base[0]) and np.array_equal(got[1], base[1])
+        # intermediate-table layouts: chunks smaller and larger than the late rounds' tables,
+        # table-major, with per-pair, point-parallel and tail rounds all reading them
+        pr.set_schedule(64, 8, False)
+        for bad in (1, -1, 64):
+            with pytest.raises(ValueError):
+                pr.set_fold_chunk(bad)
+        for chunk in (2, 3, 7, 11, 63, 0):
+            for wide in (0, 1 << 9, -1):
+                say("fold chunk", bw, chunk, wide)
+                pr.set_fold_chunk(chunk)
+                pr.set_point_parallel(wide)
+                pr.run("sha3")
+                pr.wait()
+                got = pr.result()
+                assert np.array_equal(got[0], base[0]) and np.array_equal(got[1], base[1])
+                rf, fin = pr.final()
+                assert np.array_equal(rf, base_final[0]) and np.array_equal(fin, base_final[1])
+        pr.set_schedule(0, 0, False)
+        pr.set_point_parallel(-1)
         pr.capture("sha3")
         for k in range(3):
             say("graph replay", k)
