@@ -634,17 +634,20 @@ __device__ __forceinline__ Elem<F::L> previous_claim(const ProofState* st, int r
 
 // g(d) of a degree-d >= 2 polynomial from g(0..d-1) and its leading coefficient in g[d]
 // (FieldConsts::lead), one term per lane of the calling wave; the value goes to lanes 0..7.
+// With use_g1, lane 1 takes g1 (its own register) instead of g[1].
 template <class F>
-__device__ __forceinline__ Elem<F::L> value_from_lead(const uint32_t* g, int npts, const FieldConsts* __restrict__ fc) {
+__device__ __forceinline__ Elem<F::L> value_from_lead(const uint32_t* g, int npts, const FieldConsts* __restrict__ fc,
+                                                      bool use_g1, const Elem<F::L>& g1) {
   using W = Work<F>;
   constexpr int L = F::L;
   const int t = threadIdx.x & 63;
   const int d = npts - 1;
   Elem<L> term = zero<F>();
   if (t <= d) {
+    const Elem<L> gt = (use_g1 && t == 1) ? g1 : load_elem<L>(g + t * L);
     typename W::Acc acc;
     acc.clear();
-    acc.add(W::mul(W::from_mont(g + t * L), W::from_mont(fc->lead[d][t])));
+    acc.add(W::mul(W::from_mont(gt.v), W::from_mont(fc->lead[d][t])));
     term = acc.value();
   }
   return group8_sum<F>(term);
@@ -657,33 +660,32 @@ __device__ __forceinline__ Elem<F::L> value_from_lead(const uint32_t* g, int npt
 // a degree d >= 2 g[d] holds the sum of the leading coefficients (the kernels evaluate the
 // pairs' polynomials at infinity instead of at t = d), from which g(d) follows by the finite
 // difference identity. Both values are the ones evaluation at the point would give.
+// claim_in (LDS, L words), when given, is the round's claim already computed (by a wave of the
+// caller while the others summed the partials); else the first wave computes it here.
 template <class F, int NT>
 __device__ __forceinline__ void finish_round(uint32_t* g, int npts, int round, int num_rounds, int mode,
                              const FieldConsts* __restrict__ fc, ProofState* st,
                              const uint32_t* __restrict__ explicit_r, uint32_t* msg, uint32_t* r_out,
-                             bool last_challenge = false, bool from_sums = false, bool lead = false) {
+                             bool last_challenge = false, bool from_sums = false, bool lead = false,
+                             const uint32_t* claim_in = nullptr) {
   constexpr int L = F::L;
   const int tid = threadIdx.x;
   const bool fill1 = from_sums && round > 0, filld = from_sums && lead && npts >= 3;
   if (fill1 || filld) {
-    // one wave; g(1) is written before the lanes read g for g(d)
+    // one wave: g(1) = claim - g(0) is kept in lanes 0..7 for the g(d) terms and written by lane 0
     if (tid < 64) {
+      Elem<L> g1 = zero<F>();
       if (fill1) {
-        const Elem<L> claim = previous_claim<F>(st, round, npts, fc);
-        if (tid == 0) {
-          const Elem<L> g1 = sub<F>(claim, load_elem<L>(g));
-#pragma unroll
-          for (int k = 0; k < L; ++k) g[L + k] = g1.v[k];
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const Elem<L> claim = claim_in != nullptr ? load_elem<L>(claim_in) : previous_claim<F>(st, round, npts, fc);
+        g1 = sub<F>(claim, load_elem<L>(g));
       }
-      if (filld) {
-        const Elem<L> gd = value_from_lead<F>(g, npts, fc);
-        if (tid == 0) {
+      Elem<L> gd = zero<F>();
+      if (filld) gd = value_from_lead<F>(g, npts, fc, fill1, g1);
+      if (tid == 0) {
 #pragma unroll
-          for (int k = 0; k < L; ++k) g[(npts - 1) * L + k] = gd.v[k];
+        for (int k = 0; k < L; ++k) {
+          if (fill1) g[L + k] = g1.v[k];
+          if (filld) g[(npts - 1) * L + k] = gd.v[k];
         }
       }
     }
@@ -733,24 +735,36 @@ __device__ __forceinline__ void finish_round(uint32_t* g, int npts, int round, i
 
 // Sums the per-block partials of a pass and finishes the round (g_j, r_j). Used by the
 // finalize kernel and by the last block of a pass kernel.
+// From round 1 on the last wave computes the round's claim (which needs only the previous
+// round's polynomial and challenge) into s_claim while the other waves sum the partials.
 template <class F, int PMAX>
 __device__ __forceinline__ void reduce_partials_and_finish(const uint32_t* partials, uint32_t nblocks,
                                                            int npts, int round, int num_rounds, int mode,
                                                            const FieldConsts* fc, ProofState* st,
                                                            const uint32_t* explicit_r, uint32_t* s_red,
                                                            uint32_t* s_g, uint32_t* s_msg, uint32_t* s_r,
-                                                           int scale_pow) {
+                                                           uint32_t* s_claim, int scale_pow) {
   constexpr int L = F::L;
   Elem<L> acc[PMAX];
 #pragma unroll
   for (int t = 0; t < PMAX; ++t) acc[t] = zero<F>();
-  for (uint32_t b = threadIdx.x; b < nblocks; b += BLOCK) {
+  const bool early = round > 0;
+  if (early && (threadIdx.x >> 6) == BLOCK / 64 - 1) {
+    const Elem<L> c = previous_claim<F>(st, round, npts, fc);
+    if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-    for (int t = 0; t < PMAX; ++t) {
-      if (t < npts) acc[t] = add<F>(acc[t], load_elem<L>(partials + ((uint64_t)b * MAX_POINTS + t) * L));
+      for (int k = 0; k < L; ++k) s_claim[k] = c.v[k];
+    }
+  } else {
+    const uint32_t step = early ? BLOCK - 64 : BLOCK;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += step) {
+#pragma unroll
+      for (int t = 0; t < PMAX; ++t) {
+        if (t < npts) acc[t] = add<F>(acc[t], load_elem<L>(partials + ((uint64_t)b * MAX_POINTS + t) * L));
+      }
     }
   }
-  block_reduce<F, PMAX, BLOCK>(acc, npts, s_red, s_g);
+  block_reduce<F, PMAX, BLOCK>(acc, npts, s_red, s_g);  // its barriers publish s_claim too
   if (scale_pow != 0) {  // the pass summed g / 32^scale_pow
     if (threadIdx.x < npts) {
       const Elem<L> g = mont_mul<F>(load_elem<L>(s_g + threadIdx.x * L), load_elem<L>(fc->pow32[scale_pow]));
@@ -759,7 +773,8 @@ __device__ __forceinline__ void reduce_partials_and_finish(const uint32_t* parti
     }
     __syncthreads();
   }
-  finish_round<F, BLOCK>(s_g, npts, round, num_rounds, mode, fc, st, explicit_r, s_msg, s_r, false, true, true);
+  finish_round<F, BLOCK>(s_g, npts, round, num_rounds, mode, fc, st, explicit_r, s_msg, s_r, false, true, true,
+                         early ? s_claim : nullptr);
 }
 
 template <class F, int TMAX, int PMAX, int SID>
@@ -769,6 +784,7 @@ __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
   __shared__ uint32_t s_g[PMAX * L];
   __shared__ __align__(16) uint32_t s_msg[MSG_WORDS];
   __shared__ uint32_t s_r[L];
+  __shared__ uint32_t s_claim[L];
   __shared__ int s_last;
   const CompDesc* __restrict__ cd = a.comp;
   const int T = Shape<SID>::tables(cd);
@@ -872,7 +888,7 @@ __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
   __threadfence();  // acquire the other blocks' partials
   constexpr int kScale = (F::L == 8 && SID != 0) ? static_round0_scale(SID) : 0;
   reduce_partials_and_finish<F, PMAX>(a.partials, gridDim.x, npts, a.round, a.num_rounds, a.challenge_mode,
-                                      a.fc, a.st, a.explicit_r, s_red, s_g, s_msg, s_r,
+                                      a.fc, a.st, a.explicit_r, s_red, s_g, s_msg, s_r, s_claim,
                                       a.round == 0 ? kScale : 0);
   if (threadIdx.x == 0) a.st->counter = 0u;  // ready for the next pass
 }
@@ -884,9 +900,10 @@ __global__ void __launch_bounds__(BLOCK) finalize_kernel(FinalizeArgs a) {
   __shared__ uint32_t s_g[PMAX * L];
   __shared__ __align__(16) uint32_t s_msg[MSG_WORDS];
   __shared__ uint32_t s_r[L];
+  __shared__ uint32_t s_claim[L];
   reduce_partials_and_finish<F, PMAX>(a.partials, a.nblocks, a.comp->npts, a.round, a.num_rounds,
                                       a.challenge_mode, a.fc, a.st, a.explicit_r, s_red, s_g, s_msg, s_r,
-                                      a.scale_pow);
+                                      s_claim, a.scale_pow);
 }
 
 // One workgroup of TB threads (more than a pass block: the tail's rounds have no other
@@ -898,6 +915,7 @@ __device__ __forceinline__ void tail_rounds(const TailArgs& a) {
   __shared__ uint32_t s_g[PMAX * L];
   __shared__ __align__(16) uint32_t s_msg[MSG_WORDS];
   __shared__ uint32_t s_r[L];
+  __shared__ uint32_t s_claim[L];
   const CompDesc* __restrict__ cd = a.comp;
   const int T = Shape<SID>::tables(cd);
   const int K = Shape<SID>::terms(cd);
@@ -918,17 +936,28 @@ __device__ __forceinline__ void tail_rounds(const TailArgs& a) {
     typename W::Acc acc[PMAX];
 #pragma unroll
     for (int t = 0; t < PMAX; ++t) acc[t].clear();
-    for (uint64_t i = threadIdx.x; i < pairs; i += TB) {
-      typename W::V cur[TMAX], oth[TMAX];
-      load_pair<F, TMAX>(cur, oth, src, dst, sm, dm, i, T, fold, j <= 1, r);
-      eval_points<F, TMAX, PMAX, SID, false>(cur, oth, acc, cd, T, K, npts, fold);
+    // a round whose pairs leave the last wave idle has it compute the claim meanwhile (from the
+    // previous round's polynomial and challenge, written before the last barrier)
+    const bool early = fold && pairs <= (uint64_t)(TB - 64);
+    if (early && (threadIdx.x >> 6) == TB / 64 - 1) {
+      const Elem<L> c = previous_claim<F>(a.st, j, npts, a.fc);
+      if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < L; ++k) s_claim[k] = c.v[k];
+      }
+    } else {
+      for (uint64_t i = threadIdx.x; i < pairs; i += TB) {
+        typename W::V cur[TMAX], oth[TMAX];
+        load_pair<F, TMAX>(cur, oth, src, dst, sm, dm, i, T, fold, j <= 1, r);
+        eval_points<F, TMAX, PMAX, SID, false>(cur, oth, acc, cd, T, K, npts, fold);
+      }
     }
     Elem<L> sums[PMAX];
 #pragma unroll
     for (int t = 0; t < PMAX; ++t) sums[t] = acc[t].value();
-    block_reduce<F, PMAX, TB>(sums, npts, s_red, s_g);
+    block_reduce<F, PMAX, TB>(sums, npts, s_red, s_g);  // its barriers publish s_claim too
     finish_round<F, TB>(s_g, npts, j, a.num_rounds, a.challenge_mode, a.fc, a.st, a.explicit_r, s_msg, s_r,
-                        a.want_final != 0, true, false);
+                        a.want_final != 0, true, false, early ? s_claim : nullptr);
     __syncthreads();
   }
   if (!a.want_final) return;
@@ -1293,20 +1322,31 @@ __global__ void __launch_bounds__(BLOCK) batch_finalize_kernel(BatchArgs a) {
   __shared__ uint32_t s_r[L];
   const StageEntry ent = a.entries[blockIdx.x];
   const InstDesc& in = a.insts[ent.inst];
+  __shared__ uint32_t s_claim[L];
   const int npts = in.comp->npts;
   Elem<L> acc[PMAX];
 #pragma unroll
   for (int t = 0; t < PMAX; ++t) acc[t] = zero<F>();
-  for (uint32_t b = threadIdx.x; b < ent.tiles; b += BLOCK) {
-    const uint32_t* part = a.partials + (uint64_t)(ent.tile_begin + b) * (MAX_POINTS * MAX_LIMBS);
+  const bool early = ent.round > 0;  // the last wave computes the claim meanwhile
+  if (early && (threadIdx.x >> 6) == BLOCK / 64 - 1) {
+    const Elem<L> c = previous_claim<F>(in.st, (int)ent.round, npts, a.fc);
+    if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-    for (int t = 0; t < PMAX; ++t) {
-      if (t < npts) acc[t] = add<F>(acc[t], load_elem<L>(part + t * L));
+      for (int k = 0; k < L; ++k) s_claim[k] = c.v[k];
+    }
+  } else {
+    const uint32_t step = early ? BLOCK - 64 : BLOCK;
+    for (uint32_t b = threadIdx.x; b < ent.tiles; b += step) {
+      const uint32_t* part = a.partials + (uint64_t)(ent.tile_begin + b) * (MAX_POINTS * MAX_LIMBS);
+#pragma unroll
+      for (int t = 0; t < PMAX; ++t) {
+        if (t < npts) acc[t] = add<F>(acc[t], load_elem<L>(part + t * L));
+      }
     }
   }
   block_reduce<F, PMAX, BLOCK>(acc, npts, s_red, s_g);
   finish_round<F, BLOCK>(s_g, npts, (int)ent.round, in.num_vars, in.challenge_mode, a.fc, in.st, nullptr,
-                         s_msg, s_r, false, true, true);
+                         s_msg, s_r, false, true, true, early ? s_claim : nullptr);
 }
 
 // Input generation and Montgomery encoding (cuda/src/field_sumcheck.cu:593-611).
