@@ -55,12 +55,18 @@ __device__ __forceinline__ void table_for(Fn&& fn) {
 // ---------------------------------------------------------------------------------------------
 // Vector memory helpers (element-major: NW consecutive words per thread)
 // ---------------------------------------------------------------------------------------------
+// The loads use clang's native vector types: with HIP's uint4 struct the backend kept a
+// prefetched 16-byte load for its upper half and loaded the lower half a second time (8 bytes)
+// where the kernel held the words across a loop iteration.
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+
 template <int NW>
 __device__ __forceinline__ void load_words(uint32_t (&w)[NW], const uint32_t* __restrict__ p) {
   if constexpr (NW % 4 == 0) {
 #pragma unroll
     for (int i = 0; i < NW / 4; ++i) {
-      const uint4 x = reinterpret_cast<const uint4*>(p)[i];
+      const u32x4_t x = reinterpret_cast<const u32x4_t*>(p)[i];
       w[4 * i + 0] = x.x;
       w[4 * i + 1] = x.y;
       w[4 * i + 2] = x.z;
@@ -68,7 +74,7 @@ __device__ __forceinline__ void load_words(uint32_t (&w)[NW], const uint32_t* __
     }
   } else {
     static_assert(NW == 2, "pairs of 1-limb entries");
-    const uint2 x = *reinterpret_cast<const uint2*>(p);
+    const u32x2_t x = *reinterpret_cast<const u32x2_t*>(p);
     w[0] = x.x;
     w[1] = x.y;
   }
@@ -809,23 +815,17 @@ __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
   // grouped loads measured no faster at 32 bits and slower at 64 (Goldilocks round 1 798 ->
   // 825 us).
   if (L <= 4 && !fold) {
-    // Round 0 only reads: the next pair's loads are issued before this pair is evaluated,
-    // so a wave's evaluation overlaps its own next memory round trip (Goldilocks num_vars 26
-    // proof 2.48 -> 2.45 ms, 2^32-5 1.38 -> 1.35 ms). Unrolling by two, so the two register
-    // sets alternate instead of being copied, measured slower (Goldilocks round 0 0.53 ->
-    // 0.59 ms, same box).
-    uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    typename W::V cur[TMAX], oth[TMAX];
-    if (i < a.pairs) load_pair_fixed<F, TMAX, false, true>(cur, oth, a.src, a.dst, sm, dm, i, T, r);
-    for (; i < a.pairs; i += stride) {
-      typename W::V ncur[TMAX], noth[TMAX];
-      if (i + stride < a.pairs) load_pair_fixed<F, TMAX, false, true>(ncur, noth, a.src, a.dst, sm, dm, i + stride, T, r);
+    // Round 0 only reads: one pair per iteration, loads then evaluation, the other waves of the
+    // SIMD (four to five) covering each wave's memory round trip. Same-box timings on MI355X
+    // (Goldilocks num_vars 26 round 0): 488-493 us this way, 502-514 us with two pairs per
+    // iteration (the second pair's loads in flight during the first pair's evaluation, kept in
+    // that order by scheduling barriers); a one-ahead prefetch held across iterations made the
+    // compiler's load waits stall the evaluation. Round 0 reads the input (table-major).
+    const TableMap im = strided_map(a.src_stride);
+    for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.pairs; i += stride) {
+      typename W::V cur[TMAX], oth[TMAX];
+      load_pair_fixed<F, TMAX, false, true>(cur, oth, a.src, a.dst, im, dm, i, T, r);
       eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts, false);
-#pragma unroll
-      for (int v = 0; v < TMAX; ++v) {
-        cur[v] = ncur[v];
-        oth[v] = noth[v];
-      }
     }
   } else if constexpr (L == 4) {
     auto sweep = [&](auto fold_c, auto mont_c) {
