@@ -383,15 +383,46 @@ __device__ __forceinline__ void eval_points(typename Work<F>::V (&cur)[TMAX], ty
   });
 }
 
+// Fields that fold by a table of the round's challenge (Work::kFoldTable: the 255-bit one)
+// build it per block in LDS: row k = r 2^(29k + S) mod p, S = 5 for folds straight from the
+// caller's Montgomery input. fold_table_words is the LDS size to declare (a placeholder for the
+// other fields). build_fold_table is called by the whole block and ends with a barrier.
+template <class F>
+constexpr int fold_table_words() {
+  if constexpr (Work<F>::kFoldTable) {
+    return Work<F>::kFoldTableWords;
+  } else {
+    return 4;
+  }
+}
+
+template <class F>
+__device__ __forceinline__ void build_fold_table(uint32_t* s_tab, const uint32_t* rm, bool src_mont) {
+  using W = Work<F>;
+  if constexpr (W::kFoldTable) {
+    if (threadIdx.x < 9) {
+      if (src_mont) {
+        W::template fold_row<5>(s_tab, rm, (int)threadIdx.x);
+      } else {
+        W::template fold_row<0>(s_tab, rm, (int)threadIdx.x);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Loads pair i of the current round for every table, folding from the previous round's
 // tables when fold != 0 (f' = e + r (o - e), field_sumcheck.cu:677-694), and writes the
 // folded entries to dst. src holds the caller's Montgomery tables when src_mont (rounds 0
 // and 1), else entries written by an earlier fold. sm and dm place the entries of src and dst.
-template <class F, int TMAX>
+// With USE_TAB (fields with Work::kFoldTable) the folds use the block's fold table tab
+// (build_fold_table) instead of r.
+template <class F, int TMAX, bool USE_TAB = false>
 __device__ __forceinline__ void load_pair(typename Work<F>::V (&cur)[TMAX], typename Work<F>::V (&oth)[TMAX],
                                           const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
                                           const TableMap& sm, const TableMap& dm, uint64_t i, int T, bool fold,
-                                          bool src_mont, const typename Work<F>::V& r) {
+                                          bool src_mont, const typename Work<F>::V& r,
+                                          const uint32_t* tab = nullptr) {
   using W = Work<F>;
   constexpr int L = F::L;
   table_for<TMAX, (TMAX <= 8)>([&](auto vi) {
@@ -400,11 +431,20 @@ __device__ __forceinline__ void load_pair(typename Work<F>::V (&cur)[TMAX], type
       if (fold) {
         uint32_t w[4 * L];
         load_words<4 * L>(w, src + sm.at(v, 4 * i) * L);
-        typename W::V f[4];
+        typename W::V e, o;
+        if constexpr (USE_TAB && W::kFoldTable) {
+          if (src_mont) {
+            W::template fold2_tab<true>(w, tab, e, o);
+          } else {
+            W::template fold2_tab<false>(w, tab, e, o);
+          }
+        } else {
+          typename W::V f[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) f[k] = src_mont ? W::from_mont(w + k * L) : W::from_stored(w + k * L);
-        const typename W::V e = W::fold(f[0], f[1], r);
-        const typename W::V o = W::fold(f[2], f[3], r);
+          for (int k = 0; k < 4; ++k) f[k] = src_mont ? W::from_mont(w + k * L) : W::from_stored(w + k * L);
+          e = W::fold(f[0], f[1], r);
+          o = W::fold(f[2], f[3], r);
+        }
         uint32_t y[2 * L];
         W::store(y, e);
         W::store(y + L, o);
@@ -475,7 +515,8 @@ template <class F, int TMAX, bool FOLD, bool SRC_MONT, int SID>
 __device__ __forceinline__ void load_pair_pipelined(typename Work<F>::V (&cur)[TMAX], typename Work<F>::V (&oth)[TMAX],
                                                     const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
                                                     const TableMap& sm, const TableMap& dm, uint64_t i, int T,
-                                                    const typename Work<F>::V& r, const typename Work<F>::V& r32) {
+                                                    const typename Work<F>::V& r, const typename Work<F>::V& r32,
+                                                    const uint32_t* tab) {
   using W = Work<F>;
   constexpr int L = F::L;
   constexpr int NW = FOLD ? 4 * L : 2 * L;
@@ -489,7 +530,9 @@ __device__ __forceinline__ void load_pair_pipelined(typename Work<F>::V (&cur)[T
       const uint32_t* wv = w[v & 1];
       if constexpr (FOLD) {
         typename W::V e, o;
-        if constexpr (SRC_MONT && W::kCheapMontInput) {
+        if constexpr (W::kFoldTable) {
+          W::template fold2_tab<SRC_MONT>(wv, tab, e, o);
+        } else if constexpr (SRC_MONT && W::kCheapMontInput) {
           e = W::fold_mont(wv, wv + L, r32);
           o = W::fold_mont(wv + 2 * L, wv + 3 * L, r32);
         } else {
@@ -783,8 +826,17 @@ __device__ __forceinline__ void reduce_partials_and_finish(const uint32_t* parti
                          early ? s_claim : nullptr);
 }
 
+// Waves per SIMD the compiler must leave room for: the compiled 255-bit compositions need
+// about 257 registers per lane for the table folds (one wave per SIMD) unless held to two
+// waves, which costs two spilled registers.
+template <class F, int SID>
+constexpr int pass_min_waves() {
+  return (F::L == 8 && SID != 0) ? 2 : 1;
+}
+
 template <class F, int TMAX, int PMAX, int SID>
-__global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(pass_min_waves<F, SID>())))
+pass_kernel(PassArgs a) {
   constexpr int L = F::L;
   __shared__ uint32_t s_red[(BLOCK / 64) * PMAX * L];
   __shared__ uint32_t s_g[PMAX * L];
@@ -844,12 +896,16 @@ __global__ void __launch_bounds__(BLOCK) pass_kernel(PassArgs a) {
   } else if constexpr (L == 8) {
     // The 255-bit field: one loop per round kind, loads one table ahead (round 0 measured 7%
     // faster, rounds 1-3 5-6%, a*b*c + d*e num_vars 24 on MI355X).
-    const typename W::V r32 = W::kCheapMontInput && fold && src_mont ? W::times32(r) : r;
+    // Folds multiply by the round's challenge through the block's fold table (no Montgomery
+    // reduction; Work<FieldP256>::fold2_tab).
+    __shared__ __align__(16) uint32_t s_tab[fold_table_words<F>()];
+    if (fold) build_fold_table<F>(s_tab, a.st->r[a.round - 1], src_mont);
+    const typename W::V r32 = W::kCheapMontInput && fold && src_mont && !W::kFoldTable ? W::times32(r) : r;
     auto sweep = [&](auto fold_c, auto mont_c) {
       constexpr bool FOLD = decltype(fold_c)::value, MONT = decltype(mont_c)::value;
       for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.pairs; i += stride) {
         typename W::V cur[TMAX], oth[TMAX];
-        load_pair_pipelined<F, TMAX, FOLD, MONT, SID>(cur, oth, a.src, a.dst, sm, dm, i, T, r, r32);
+        load_pair_pipelined<F, TMAX, FOLD, MONT, SID>(cur, oth, a.src, a.dst, sm, dm, i, T, r, r32, s_tab);
         eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts, FOLD);
       }
     };
@@ -1007,7 +1063,7 @@ template <class F>
 __device__ __forceinline__ void wide_fold_range(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
                                                 uint64_t src_n, const TableMap& sm, const TableMap& dm,
                                                 bool src_mont, const typename Work<F>::V& r, uint64_t begin,
-                                                uint64_t end, uint64_t stride) {
+                                                uint64_t end, uint64_t stride, const uint32_t* tab) {
   using W = Work<F>;
   constexpr int L = F::L;
   const uint64_t n_new = src_n >> 1;  // a power of two
@@ -1016,10 +1072,14 @@ __device__ __forceinline__ void wide_fold_range(const uint32_t* __restrict__ src
     const uint64_t v = idx >> shift, i = idx & (n_new - 1);
     uint32_t w[2 * L];
     load_words<2 * L>(w, src + sm.at(v, 2 * i) * L);
-    const typename W::V lo = src_mont ? W::from_mont(w) : W::from_stored(w);
-    const typename W::V hi = src_mont ? W::from_mont(w + L) : W::from_stored(w + L);
     uint32_t y[L];
-    W::store(y, W::fold(lo, hi, r));
+    if constexpr (W::kFoldTable) {
+      W::store(y, src_mont ? W::template fold1_tab<true>(w, w + L, tab) : W::template fold1_tab<false>(w, w + L, tab));
+    } else {
+      const typename W::V lo = src_mont ? W::from_mont(w) : W::from_stored(w);
+      const typename W::V hi = src_mont ? W::from_mont(w + L) : W::from_stored(w + L);
+      W::store(y, W::fold(lo, hi, r));
+    }
     uint32_t* d = dst + dm.at(v, i) * L;
 #pragma unroll
     for (int k = 0; k < L; ++k) d[k] = y[k];
@@ -1168,12 +1228,14 @@ __device__ __forceinline__ void wide_eval_block(const uint32_t* __restrict__ src
 
 template <class F>
 __global__ void __launch_bounds__(BLOCK) wide_fold_kernel(PassArgs a) {
+  __shared__ __align__(16) uint32_t s_tab[fold_table_words<F>()];
+  build_fold_table<F>(s_tab, a.st->r[a.round - 1], a.round <= 1);
   const int T = a.comp->num_tables;
   const uint64_t total = (a.src_n >> 1) * (uint64_t)T;
   wide_fold_range<F>(a.src, a.dst, a.src_n, source_map(a.round, a.src_stride, a.src_n, T, a.chunk_shift),
                      fold_map(a.src_n >> 1, T, a.chunk_shift),
                      a.round <= 1, Work<F>::from_mont(a.st->r[a.round - 1]),
-                     (uint64_t)blockIdx.x * BLOCK + threadIdx.x, total, (uint64_t)gridDim.x * BLOCK);
+                     (uint64_t)blockIdx.x * BLOCK + threadIdx.x, total, (uint64_t)gridDim.x * BLOCK, s_tab);
 }
 
 template <class F, int TMAX, int G>
@@ -1224,6 +1286,8 @@ __global__ void __launch_bounds__(BLOCK) batch_pass_kernel(BatchArgs a) {
   const TableMap dm = fold_map(src_n >> 1, T, a.chunk_shift);
   using W = Work<F>;
   const typename W::V r = W::from_mont(in.st->r[fold ? j - 1 : 0]);  // unused in round 0
+  __shared__ __align__(16) uint32_t s_tab[fold_table_words<F>()];
+  if (fold) build_fold_table<F>(s_tab, in.st->r[j - 1], j <= 1);
 
   typename W::Acc acc[PMAX];
 #pragma unroll
@@ -1234,7 +1298,7 @@ __global__ void __launch_bounds__(BLOCK) batch_pass_kernel(BatchArgs a) {
     const uint64_t i = base + (uint64_t)it * BLOCK + threadIdx.x;
     if (i < pairs) {
       typename W::V cur[TMAX], oth[TMAX];
-      load_pair<F, TMAX>(cur, oth, src, dst, sm, dm, i, T, fold, j <= 1, r);
+      load_pair<F, TMAX, true>(cur, oth, src, dst, sm, dm, i, T, fold, j <= 1, r, s_tab);
       eval_points<F, TMAX, PMAX, 0>(cur, oth, acc, cd, T, K, npts, fold);
     }
   }
@@ -1262,10 +1326,12 @@ __global__ void __launch_bounds__(BLOCK) batch_wide_fold_kernel(BatchArgs a) {
   const uint64_t total = (src_n >> 1) * (uint64_t)T;
   const uint64_t span = (uint64_t)BLOCK * ent.items;
   const uint64_t begin = (uint64_t)(blockIdx.x - ent.tile_begin) * span;
+  __shared__ __align__(16) uint32_t s_tab[fold_table_words<F>()];
+  build_fold_table<F>(s_tab, in.st->r[j - 1], j <= 1);
   wide_fold_range<F>(src, dst, src_n, source_map(j, in.n0, src_n, T, a.chunk_shift),
                      fold_map(src_n >> 1, T, a.chunk_shift), j <= 1,
                      Work<F>::from_mont(in.st->r[j - 1]), begin + threadIdx.x,
-                     begin + span < total ? begin + span : total, BLOCK);
+                     begin + span < total ? begin + span : total, BLOCK, s_tab);
 }
 
 // Evaluation: block b (tile index b + tile_base in the stage's partials) evaluates

@@ -22,6 +22,8 @@
 // never overflow a 64-bit column sum, so products need no carry chains.
 #pragma once
 
+#include <math.h>
+
 #include "field.hpp"
 
 namespace sc {
@@ -33,6 +35,8 @@ struct Work {
   // The 255-bit form reads Montgomery input more cheaply in two places (Work<FieldP256>):
   // folding straight from it, and reading it scaled in round 0 of compiled compositions.
   static constexpr bool kCheapMontInput = false;
+  // The 255-bit form folds by a per-block table of the challenge (fold2_tab, fold1_tab).
+  static constexpr bool kFoldTable = false;
   static constexpr bool kLazyStep = false;  // step_lazy: an interpolation step left unnormalized
   using V = Elem<L>;
   SC_HD static V load(const uint32_t* w) {
@@ -197,6 +201,7 @@ struct Work<FieldP64> {
   static constexpr int L = 2;
   static constexpr bool kMontIsStored = true;
   static constexpr bool kCheapMontInput = false;
+  static constexpr bool kFoldTable = false;
   static constexpr bool kLazyStep = false;
   using V = Elem<2>;
   SC_HD static V load(const uint32_t* w) {
@@ -466,6 +471,52 @@ SC_HD R29 r29_mul_sum(const R29& a1, const R29& b1, const R29& a2, const R29& b2
   return r;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Folding by the round's challenge without a Montgomery reduction. Every fold of a round
+// multiplies by the same r, so for D = sum_k D_k 2^(29k)
+//   r D = sum_k D_k (r 2^(29k) mod p)   (mod p):
+// nine rows of nine limbs (the fold table, built once per block from r, fold_table_row) and 81
+// multiply-adds into nine 64-bit columns, the base value (lo) added in. With D_k < 2^31 and the
+// table canonical, each column stays below 2^63.2 and the sum V below (2^34.2 + 33) p. The
+// quotient q = floor(V / p - 2^-16) comes from the top columns in double precision: the
+// terms dropped (column 7's low word and below) and the rounding of three fused multiply-adds
+// move the estimate by less than 2^-16.6 down and 2^-17.1 up, so 0 <= V - q p < p (1 + 2^-14.9).
+// V - q p is formed as V + q (2^261 - p) with the bits from 2^261 up, which hold exactly q,
+// dropped: 17 multiply-adds and one carry pass. The result has normalized limbs and a value
+// below p + 2^240 (r29_reduce's output bound, p + 2^241, holds).
+// ---------------------------------------------------------------------------------------------
+namespace r29 {
+constexpr int TAB_STRIDE = 12;  // words per fold-table row: rows start 16-byte aligned
+constexpr double K264 = 0x1.1aa84a76ff6f2p+9;   // 2^264 / p, rounded to nearest
+constexpr double K232 = 0x1.1aa84a76ff6f2p-23;  // 2^232 / p
+constexpr double K235 = 0x1.1aa84a76ff6f2p-20;  // 2^235 / p
+}  // namespace r29
+
+// Columns c (value V = sum_i c_i 2^(29i), bounds above) -> V mod p as a normalized R29 below
+// p + 2^240.
+SC_HD R29 r29_fold_finish(uint64_t (&c)[9]) {
+  const double h8 = (double)(uint32_t)(c[8] >> 32), l8 = (double)(uint32_t)c[8];
+  const double h7 = (double)(uint32_t)(c[7] >> 32);
+  double q = fma(h7, r29::K235, -0x1p-16);
+  q = fma(l8, r29::K232, q);
+  q = fma(h8, r29::K264, q);
+  q = floor(fmax(q, 0.0));
+  const double q1d = floor(q * 0x1p-29);  // q < 2^34.3: q = q1 2^29 + q0, both exact
+  const uint32_t q1 = (uint32_t)q1d, q0 = (uint32_t)fma(q1d, -0x1p29, q);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) c[i] += (uint64_t)q0 * r29::NP(i);
+#pragma unroll
+  for (int i = 1; i < 9; ++i) c[i] += (uint64_t)q1 * r29::NP(i - 1);
+  R29 x;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    c[i + 1] += c[i] >> 29;
+    x.l[i] = (uint32_t)c[i] & r29::MASK;
+  }
+  x.l[8] = (uint32_t)c[8] & r29::MASK;
+  return x;
+}
+
 // Eight 32-bit words W -> limbs of W * 2^S (S = 0: fold buffers; S = 5: Montgomery input,
 // turning X = x 2^256 into 32 X = x 2^261).
 template <int S>
@@ -522,13 +573,92 @@ SC_HD Elem<8> r29_to_mont(const R29& x) {
   return y;
 }
 
+// Row k of the fold table of the challenge whose Montgomery words are rm: the limbs of
+// r 2^(29k + S) mod p, canonical. S = 5 for folds straight from the caller's Montgomery input,
+// whose working value is 32 X.
+template <int S>
+SC_HD void fold_table_row(uint32_t* row, const uint32_t* rm, int k) {
+  Elem<8> x, c;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) x.v[i] = rm[i];
+  const int e = 29 * k + S;  // below 254, so 2^e < p
+#pragma unroll
+  for (int i = 0; i < 8; ++i) c.v[i] = i == (e >> 5) ? 1u << (e & 31) : 0u;  // no indexed register array
+  const Elem<8> t = mont_mul<FieldP256>(x, c);  // X 2^e 2^-256 = r 2^e
+  const R29 l = r29_unpack<0>(t.v);
+#pragma unroll
+  for (int j = 0; j < 9; ++j) row[j] = l.l[j];
+}
+
 template <>
 struct Work<FieldP256> {
   static constexpr int L = 8;
   static constexpr bool kMontIsStored = false;
   static constexpr bool kCheapMontInput = true;
+  static constexpr bool kFoldTable = true;
   static constexpr bool kLazyStep = true;
   using V = R29;
+  // Words per fold table (9 rows of r29::TAB_STRIDE), and its rows for a challenge: row k of
+  // the table the folds of stored entries use (S = 0) or of folds straight from the caller's
+  // Montgomery input (S = 5).
+  static constexpr int kFoldTableWords = 9 * r29::TAB_STRIDE;
+  template <int S>
+  SC_HD static void fold_row(uint32_t* tab, const uint32_t* rm, int k) {
+    fold_table_row<S>(tab + k * r29::TAB_STRIDE, rm, k);
+  }
+  // One fold lo + r (hi - lo) of words lo_w, hi_w (as for fold2_tab), r given by its fold table.
+  template <bool MONT>
+  SC_HD static V fold1_tab(const uint32_t* lo_w, const uint32_t* hi_w, const uint32_t* tab) {
+    uint32_t D[1][9];
+    uint64_t c[1][9];
+    const R29 lo = r29_unpack<0>(lo_w), hi = r29_unpack<0>(hi_w);
+    const R29 base = MONT ? r29_unpack<5>(lo_w) : lo;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      D[0][i] = hi.l[i] + r29::PAD(3, i) - lo.l[i];
+      c[0][i] = base.l[i];
+    }
+    tab_columns<1>(c, D, tab);
+    return r29_fold_finish(c[0]);
+  }
+  // Both folded entries of one table of a pair, e = w0 + r (w1 - w0) and o = w2 + r (w3 - w2),
+  // from its four entries' words w (stored working values, or with MONT the caller's
+  // Montgomery words), r given by its fold table tab (16-byte aligned). One fold after the
+  // other: folding both in one pass over the table rows reads the rows once, but its live
+  // columns took the a*b*c + d*e pass kernel from 231 to 256 VGPRs (one wave per SIMD).
+  template <bool MONT>
+  SC_HD static void fold2_tab(const uint32_t* w, const uint32_t* tab, V& e, V& o) {
+    e = fold1_tab<MONT>(w, w + 8, tab);
+    o = fold1_tab<MONT>(w + 16, w + 24, tab);
+  }
+  // c[n][i] += sum_k D[n][k] row_k[i]. On the device the table sits in LDS; its address is
+  // passed through an empty asm statement per call so the compiler re-reads the rows at each
+  // fold instead of hoisting all 81 words into registers for the whole loop.
+  template <int N>
+  SC_HD static void tab_columns(uint64_t (&c)[N][9], const uint32_t (&D)[N][9], const uint32_t* tab) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t z = 0;
+    asm volatile("" : "+v"(z));
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4* t4 = reinterpret_cast<const u32x4*>(tab) + z;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const u32x4 a = t4[3 * k], b = t4[3 * k + 1], cc = t4[3 * k + 2];
+      const uint32_t row[9] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, cc.x};
+#pragma unroll
+      for (int n = 0; n < N; ++n) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) c[n][i] += (uint64_t)D[n][k] * row[i];
+      }
+    }
+#else
+    for (int k = 0; k < 9; ++k) {
+      for (int n = 0; n < N; ++n) {
+        for (int i = 0; i < 9; ++i) c[n][i] += (uint64_t)D[n][k] * tab[k * r29::TAB_STRIDE + i];
+      }
+    }
+#endif
+  }
   SC_HD static V from_mont(const uint32_t* w) { return r29_reduce(r29_unpack<5>(w)); }
   // The caller's Montgomery X = x 2^256 < p taken as the working value itself, which stands
   // for the residue x 2^-5 (no shift, no reduction; bounds as a stored value).
@@ -759,6 +889,7 @@ struct Work<FieldP128> {
   static constexpr int L = 4;
   static constexpr bool kMontIsStored = false;
   static constexpr bool kCheapMontInput = false;
+  static constexpr bool kFoldTable = false;
   static constexpr bool kLazyStep = false;
   using V = R26;
   // Montgomery X = 2x (X < p): x is X rotated right by one within 127 bits.

@@ -167,6 +167,41 @@ extern "C" void r29_host(int op, uint64_t n, const uint32_t* a, uint32_t* out) {
 }
 
 // Bit interleaving of the wave Keccak (csrc/keccak.hpp): the halves of each word and the join.
+// 255-bit fold by the challenge's fold table (Work<FieldP256>::fold2_tab / fold1_tab) on raw
+// words: lo, hi hold 8 words per entry (stored working values, or with mont != 0 the caller's
+// Montgomery words), rm the challenge's Montgomery words; out gets the result's nine limbs.
+// Entries are taken two at a time (fold2_tab); an odd last one goes through fold1_tab.
+extern "C" void r29_fold_tab_host(int mont, uint64_t n, const uint32_t* lo, const uint32_t* hi, const uint32_t* rm,
+                                  uint32_t* out) {
+  using W = Work<FieldP256>;
+  uint32_t tab[W::kFoldTableWords];
+  for (int k = 0; k < 9; ++k) {
+    if (mont) W::fold_row<5>(tab, rm, k);
+    else W::fold_row<0>(tab, rm, k);
+  }
+  uint64_t i = 0;
+  for (; i + 1 < n; i += 2) {
+    uint32_t w[32];
+    for (int k = 0; k < 8; ++k) {
+      w[k] = lo[i * 8 + k];
+      w[8 + k] = hi[i * 8 + k];
+      w[16 + k] = lo[(i + 1) * 8 + k];
+      w[24 + k] = hi[(i + 1) * 8 + k];
+    }
+    R29 e, o;
+    if (mont) W::fold2_tab<true>(w, tab, e, o);
+    else W::fold2_tab<false>(w, tab, e, o);
+    for (int k = 0; k < 9; ++k) {
+      out[i * 9 + k] = e.l[k];
+      out[(i + 1) * 9 + k] = o.l[k];
+    }
+  }
+  if (i < n) {
+    const R29 e = mont ? W::fold1_tab<true>(lo + i * 8, hi + i * 8, tab) : W::fold1_tab<false>(lo + i * 8, hi + i * 8, tab);
+    for (int k = 0; k < 9; ++k) out[i * 9 + k] = e.l[k];
+  }
+}
+
 extern "C" void interleave_host(uint64_t n, const uint64_t* w, uint32_t* even, uint32_t* odd, uint64_t* joined) {
   for (uint64_t i = 0; i < n; ++i) {
     even[i] = interleaved_half(w[i], 0);

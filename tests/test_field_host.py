@@ -35,6 +35,8 @@ def harness(tmp_path_factory):
     u64p = ctypes.POINTER(ctypes.c_uint64)
     lib.gl_host_mul_sum.argtypes = [ctypes.c_uint64, u64p, u64p, u64p, u64p, u64p]
     lib.gl_host_mul_sum.restype = None
+    lib.r29_fold_tab_host.argtypes = [ctypes.c_int, ctypes.c_uint64, u32p, u32p, u32p, u32p]
+    lib.r29_fold_tab_host.restype = None
     lib.interleave_host.argtypes = [ctypes.c_uint64, u64p, u32p, u32p, u64p]
     lib.interleave_host.restype = None
     lib.gl_host_accum.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32), u64p, u64p]
@@ -95,6 +97,16 @@ def harness(tmp_path_factory):
         return [int(v) for v in out]
 
     op.gl_fast = gl_fast
+
+    def fold_tab(mont, lo, hi, rm):
+        lo, hi = (np.ascontiguousarray(x, dtype=np.uint32) for x in (lo, hi))
+        rm = np.ascontiguousarray(rm, dtype=np.uint32)
+        out = np.zeros((lo.shape[0], 9), dtype=np.uint32)
+        lib.r29_fold_tab_host(mont, lo.shape[0], lo.ctypes.data_as(u32p), hi.ctypes.data_as(u32p),
+                              rm.ctypes.data_as(u32p), out.ctypes.data_as(u32p))
+        return out
+
+    op.fold_tab = fold_tab
 
     op.gl_mul_sum = gl_mul_sum
     op.gl_accum = gl_accum
@@ -209,6 +221,51 @@ def test_host_r29_pack_and_reduce(harness):
         v = sum(int(x) << (29 * i) for i, x in enumerate(row_in))
         r = sum(int(x) << (29 * i) for i, x in enumerate(row_out))
         assert r % p == v % p and r < p + (1 << 241) and all(x < (1 << 29) for x in row_out)
+
+
+def _words(x):
+    return [(x >> (32 * k)) & 0xFFFFFFFF for k in range(8)]
+
+
+def test_host_r29_fold_by_table(harness):
+    """The 255-bit fold by the challenge's table (no Montgomery reduction; the quotient from
+    the top columns in double precision) on stored values across their whole range
+    [0, p + 2^241), edge cases included, and on the caller's Montgomery input: the result is
+    congruent to lo + r (hi - lo) in the working form (x 2^261), below p + 2^241, with every limb
+    below 2^29. Also values placed next to multiples of p, where a wrong quotient would show."""
+    p = get_field(256).modulus
+    W = 1 << 261
+    top = p + (1 << 241)
+    rng = random.Random(11)
+    rs = [1, 2, p - 1, (p + 1) // 2, 1 << 200] + [rng.randrange(p) for _ in range(6)]
+    for r in rs:
+        rm = _words(r * (1 << 256) % p)
+        lo = [0, 0, top - 1, top - 1, p, p - 1, 1] + [rng.randrange(top) for _ in range(1500)]
+        hi = [top - 1, 0, 0, top - 1, 0, p, top - 2] + [rng.randrange(top) for _ in range(1500)]  # odd count: the last one takes fold1_tab
+        # hi within 2 of lo: D = hi - lo + 3p close to 3p
+        for _ in range(200):
+            x = rng.randrange(p)
+            y = (x + rng.choice([-2, -1, 0, 1, 2])) % p
+            lo.append(x)
+            hi.append(y)
+        out = harness.fold_tab(0, [_words(x) for x in lo], [_words(x) for x in hi], rm)
+        for a, b, row in zip(lo, hi, out.tolist()):
+            v = sum(int(x) << (29 * i) for i, x in enumerate(row))
+            assert all(x < (1 << 29) for x in row), (r, a, b)
+            assert v < top, (r, a, b)
+            assert v % p == (a + r * (b - a)) % p, (r, a, b)
+        # straight from Montgomery input X = x 2^256 (canonical): working value x 2^261 = 32 X
+        mlo = [0, p - 1, p - 1] + [rng.randrange(p) for _ in range(600)]
+        mhi = [p - 1, 0, p - 1] + [rng.randrange(p) for _ in range(600)]
+        if len(mlo) % 2 == 0:  # an odd count: the last entry goes through the one-fold form
+            mlo.append(1)
+            mhi.append(2)
+        out = harness.fold_tab(1, [_words(x) for x in mlo], [_words(x) for x in mhi], rm)
+        for a, b, row in zip(mlo, mhi, out.tolist()):
+            v = sum(int(x) << (29 * i) for i, x in enumerate(row))
+            assert all(x < (1 << 29) for x in row) and v < top, (r, a, b)
+            assert v % p == 32 * (a + r * (b - a)) % p, (r, a, b)
+    assert W > top
 
 
 def test_host_keccak_bit_interleaving(harness):
