@@ -153,8 +153,8 @@ __device__ __forceinline__ TableMap source_map(int j, uint64_t in_stride, uint64
 // a scalar branch; table slots are indexed statically (unrolled v), keeping x in VGPRs.
 // Terms with fewer than min_len factors are left out (the point-parallel lane that evaluates
 // the leading coefficient passes the degree).
-template <class F, int TMAX, bool EXPAND = (TMAX <= 8)>
-__device__ __forceinline__ void compose_add(typename Work<F>::Acc& acc, const typename Work<F>::V (&x)[TMAX],
+template <class F, int TMAX, bool EXPAND = (TMAX <= 8), class A>
+__device__ __forceinline__ void compose_add(A& acc, const typename Work<F>::V (&x)[TMAX],
                                             const CompDesc* __restrict__ cd, int K, int min_len = 0) {
   using W = Work<F>;
   for (int k = 0; k < K; ++k) {
@@ -186,8 +186,8 @@ __device__ __forceinline__ void compose_add(typename Work<F>::Acc& acc, const ty
 
 // acc += a b. Goldilocks adds the raw 128-bit product to its wide accumulator (one reduction
 // per thread for the whole sum instead of one per product).
-template <class F>
-__device__ __forceinline__ void acc_add_mul(typename Work<F>::Acc& acc, const typename Work<F>::V& a,
+template <class F, class A>
+__device__ __forceinline__ void acc_add_mul(A& acc, const typename Work<F>::V& a,
                                             const typename Work<F>::V& b) {
   if constexpr (std::is_same_v<F, FieldP64>) {
     acc.add_product(a, b);
@@ -198,8 +198,8 @@ __device__ __forceinline__ void acc_add_mul(typename Work<F>::Acc& acc, const ty
 
 // Compiled composition STATIC_COMPS[SID]: every loop has constant bounds and indices, so it
 // unrolls into straight-line field operations on fixed registers.
-template <class F, int SID, int TMAX>
-__device__ __forceinline__ void compose_static(typename Work<F>::Acc& acc, const typename Work<F>::V (&x)[TMAX],
+template <class F, int SID, int TMAX, class A>
+__device__ __forceinline__ void compose_static(A& acc, const typename Work<F>::V (&x)[TMAX],
                                                const CompDesc* __restrict__ cd) {
   using W = Work<F>;
   static_assert(STATIC_COMPS[SID].num_tables <= TMAX, "table slots");
@@ -246,8 +246,8 @@ __device__ __forceinline__ void compose_static(typename Work<F>::Acc& acc, const
   });
 }
 
-template <class F, int SID, int TMAX>
-__device__ __forceinline__ void compose(typename Work<F>::Acc& acc, const typename Work<F>::V (&x)[TMAX],
+template <class F, int SID, int TMAX, class A>
+__device__ __forceinline__ void compose(A& acc, const typename Work<F>::V (&x)[TMAX],
                                         const CompDesc* __restrict__ cd, int K) {
   if constexpr (SID == 0) {
     compose_add<F, TMAX>(acc, x, cd, K);
@@ -258,8 +258,8 @@ __device__ __forceinline__ void compose(typename Work<F>::Acc& acc, const typena
 
 // The leading coefficient of a pair's polynomial C(lo + t (hi - lo)) (degree d): the terms of
 // d factors, each factor replaced by its slope s_v = hi_v - lo_v. acc += that coefficient.
-template <class F, int TMAX, bool EXPAND = (TMAX <= 8)>
-__device__ __forceinline__ void compose_add_lead(typename Work<F>::Acc& acc, const typename Work<F>::V (&s)[TMAX],
+template <class F, int TMAX, bool EXPAND = (TMAX <= 8), class A>
+__device__ __forceinline__ void compose_add_lead(A& acc, const typename Work<F>::V (&s)[TMAX],
                                                  const CompDesc* __restrict__ cd, int K, int d) {
   using W = Work<F>;
   for (int k = 0; k < K; ++k) {
@@ -289,8 +289,8 @@ __device__ __forceinline__ void compose_add_lead(typename Work<F>::Acc& acc, con
   }
 }
 
-template <class F, int SID, int TMAX>
-__device__ __forceinline__ void compose_static_lead(typename Work<F>::Acc& acc, const typename Work<F>::V (&s)[TMAX],
+template <class F, int SID, int TMAX, class A>
+__device__ __forceinline__ void compose_static_lead(A& acc, const typename Work<F>::V (&s)[TMAX],
                                                     const CompDesc* __restrict__ cd) {
   using W = Work<F>;
   constexpr int D = static_degree(SID);
@@ -315,8 +315,8 @@ __device__ __forceinline__ void compose_static_lead(typename Work<F>::Acc& acc, 
   });
 }
 
-template <class F, int SID, int TMAX>
-__device__ __forceinline__ void compose_lead(typename Work<F>::Acc& acc, const typename Work<F>::V (&s)[TMAX],
+template <class F, int SID, int TMAX, class A>
+__device__ __forceinline__ void compose_lead(A& acc, const typename Work<F>::V (&s)[TMAX],
                                              const CompDesc* __restrict__ cd, int K, int d) {
   if constexpr (SID == 0) {
     compose_add_lead<F, TMAX>(acc, s, cd, K, d);
@@ -345,9 +345,9 @@ struct Shape {
 // where the field allows, step_add), and into acc[d] either the value at t = d (LEAD false)
 // or, for d >= 2, the polynomial's leading coefficient (its value at infinity: the degree-d
 // terms over the differences), from which the round's finish recovers g(d).
-template <class F, int TMAX, int PMAX, int SID, bool LEAD = true>
+template <class F, int TMAX, int PMAX, int SID, bool LEAD = true, class A>
 __device__ __forceinline__ void eval_points(typename Work<F>::V (&cur)[TMAX], typename Work<F>::V (&oth)[TMAX],
-                                            typename Work<F>::Acc (&acc)[PMAX], const CompDesc* __restrict__ cd,
+                                            A (&acc)[PMAX], const CompDesc* __restrict__ cd,
                                             int T, int K, int npts, bool skip1) {
   using W = Work<F>;
   constexpr bool X = SID != 0 || TMAX <= 8;  // expand every loop unless the table count is large
@@ -504,24 +504,25 @@ __device__ __forceinline__ void load_pair_fixed(typename Work<F>::V (&cur)[TMAX]
 
 // The 255-bit field's per-pair loads: table v + 1's words are requested before table v is
 // converted and folded (two tables' words in flight per thread; all of them would not fit the
-// registers). Requesting the next pair's first table before this pair's evaluation, so no
-// pair starts on a whole memory round trip, measured slower (254 VGPRs; a*b*c + d*e num_vars
-// 24 folding rounds 4-10% slower, same box). FOLD / SRC_MONT fix the round kind at compile
+// registers). With PRE the caller has already requested table 0 (the folding rounds of compiled
+// compositions request the next pair's table 0 before this pair's evaluation, with their
+// running sums moved to LDS to make room; with the sums in registers that prefetch took the
+// kernel to 254 VGPRs and measured 4-10% slower). FOLD / SRC_MONT fix the round kind at compile
 // time: round 0 (SRC_MONT, no fold), round 1 (folding the caller's Montgomery input) or a
 // later round (folding stored entries). Round 0 of a compiled composition reads table v as
 // from_mont_scaled unless it is one of the factors read exactly (static_round0_exact).
-// r32 = Work::times32(r) for round 1.
-template <class F, int TMAX, bool FOLD, bool SRC_MONT, int SID>
+// r32 = Work::times32(r) for round 1. w is the caller's word buffer: with PRE, w[0] holds table
+// 0's words of pair i on entry.
+template <class F, int TMAX, bool FOLD, bool SRC_MONT, int SID, bool PRE = false>
 __device__ __forceinline__ void load_pair_pipelined(typename Work<F>::V (&cur)[TMAX], typename Work<F>::V (&oth)[TMAX],
                                                     const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
                                                     const TableMap& sm, const TableMap& dm, uint64_t i, int T,
                                                     const typename Work<F>::V& r, const typename Work<F>::V& r32,
-                                                    const uint32_t* tab) {
+                                                    const uint32_t* tab, uint32_t (&w)[2][FOLD ? 4 * F::L : 2 * F::L]) {
   using W = Work<F>;
   constexpr int L = F::L;
   constexpr int NW = FOLD ? 4 * L : 2 * L;
-  uint32_t w[2][NW];
-  load_words<NW>(w[0], src + sm.at(0, (FOLD ? 4 : 2) * i) * L);
+  if constexpr (!PRE) load_words<NW>(w[0], src + sm.at(0, (FOLD ? 4 : 2) * i) * L);
   table_for<TMAX, (TMAX <= 8)>([&](auto vi) {
     const int v = vi;
     if (v < T) {
@@ -826,9 +827,41 @@ __device__ __forceinline__ void reduce_partials_and_finish(const uint32_t* parti
                          early ? s_claim : nullptr);
 }
 
-// Waves per SIMD the compiler must leave room for: the compiled 255-bit compositions need
-// about 257 registers per lane for the table folds (one wave per SIMD) unless held to two
-// waves, which costs two spilled registers.
+// The 255-bit running sum of one point, kept in LDS between updates: limb i of this thread's
+// sum at p[i * BLOCK] (one word array per limb, so a wave's accesses hit distinct banks).
+struct AccLds256 {
+  uint32_t* p;
+  using Acc = Work<FieldP256>::Acc;
+  __device__ __forceinline__ Acc load() const {
+    Acc a;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) a.v.l[i] = p[i * BLOCK];
+    return a;
+  }
+  __device__ __forceinline__ void save(const Acc& a) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) p[i * BLOCK] = a.v.l[i];
+  }
+  __device__ __forceinline__ void clear() {
+    Acc a;
+    a.clear();
+    save(a);
+  }
+  __device__ __forceinline__ void add(const R29& x) {
+    Acc a = load();
+    a.add(x);
+    save(a);
+  }
+  __device__ __forceinline__ void sub(const R29& x) {
+    Acc a = load();
+    a.sub(x);
+    save(a);
+  }
+};
+
+// Waves per SIMD the compiler must leave room for: two for the compiled 255-bit compositions
+// (the a*b*c + d*e kernel holds 227 registers per lane; left free, the compiler may spend more
+// and fall to one wave per SIMD).
 template <class F, int SID>
 constexpr int pass_min_waves() {
   return (F::L == 8 && SID != 0) ? 2 : 1;
@@ -901,12 +934,47 @@ pass_kernel(PassArgs a) {
     __shared__ __align__(16) uint32_t s_tab[fold_table_words<F>()];
     if (fold) build_fold_table<F>(s_tab, a.st->r[a.round - 1], src_mont);
     const typename W::V r32 = W::kCheapMontInput && fold && src_mont && !W::kFoldTable ? W::times32(r) : r;
+    // Folding rounds of compiled compositions: the running sums live in LDS between pairs
+    // (AccLds256), which frees their registers (227 instead of 255 + 2 spilled) for table 0 of
+    // the thread's next pair, requested before this pair's evaluation so the wave does not
+    // start each pair on a whole memory round trip (a*b*c + d*e num_vars 24 on MI355X, same
+    // box, two A/B pairs: pass kernel 3773-3823 -> 3863-3987 GB/s). Interpreted compositions
+    // keep their sums in registers (eight points would need 72 KiB of LDS per block).
+    constexpr bool kPipe = SID != 0;
+    __shared__ uint32_t s_acc[kPipe ? PMAX * 9 * BLOCK : 1];
     auto sweep = [&](auto fold_c, auto mont_c) {
       constexpr bool FOLD = decltype(fold_c)::value, MONT = decltype(mont_c)::value;
-      for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.pairs; i += stride) {
-        typename W::V cur[TMAX], oth[TMAX];
-        load_pair_pipelined<F, TMAX, FOLD, MONT, SID>(cur, oth, a.src, a.dst, sm, dm, i, T, r, r32, s_tab);
-        eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts, FOLD);
+      constexpr int NW = FOLD ? 4 * L : 2 * L;
+      uint32_t w[2][NW];
+      if constexpr (FOLD && kPipe) {
+        static_assert(std::is_same_v<F, FieldP256>, "LDS running sums of the 255-bit form");
+        AccLds256 lacc[PMAX];
+#pragma unroll
+        for (int t = 0; t < PMAX; ++t) {
+          lacc[t].p = s_acc + t * 9 * BLOCK + threadIdx.x;
+          lacc[t].clear();
+        }
+        uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+        if (i < a.pairs) load_words<NW>(w[0], a.src + sm.at(0, 4 * i) * L);
+        for (; i < a.pairs; i += stride) {
+          typename W::V cur[TMAX], oth[TMAX];
+          load_pair_pipelined<F, TMAX, FOLD, MONT, SID, true>(cur, oth, a.src, a.dst, sm, dm, i, T, r, r32, s_tab, w);
+          if (i + stride < a.pairs) load_words<NW>(w[0], a.src + sm.at(0, 4 * (i + stride)) * L);
+          uint32_t z = 0;
+          asm volatile("" : "+v"(z));  // the sums are re-read from LDS at every pair, not hoisted
+          AccLds256 pacc[PMAX];
+#pragma unroll
+          for (int t = 0; t < PMAX; ++t) pacc[t].p = lacc[t].p + z;
+          eval_points<F, TMAX, PMAX, SID>(cur, oth, pacc, cd, T, K, npts, FOLD);
+        }
+#pragma unroll
+        for (int t = 0; t < PMAX; ++t) acc[t] = lacc[t].load();
+      } else {
+        for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < a.pairs; i += stride) {
+          typename W::V cur[TMAX], oth[TMAX];
+          load_pair_pipelined<F, TMAX, FOLD, MONT, SID>(cur, oth, a.src, a.dst, sm, dm, i, T, r, r32, s_tab, w);
+          eval_points<F, TMAX, PMAX, SID>(cur, oth, acc, cd, T, K, npts, FOLD);
+        }
       }
     };
     if (!fold) {
