@@ -16,6 +16,7 @@
 // wave diverges inside the arithmetic.
 #pragma once
 
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__)
@@ -116,6 +117,51 @@ SC_HD Elem<F::L> add_limbs(const Elem<F::L>& a, const Elem<F::L>& b) {
 #pragma unroll
   for (int i = 0; i < F::L; ++i) s.v[i] = use_t ? t.v[i] : s.v[i];
   return s;
+}
+
+// V mod p for V = sum_k col_k 2^(32k) given as L column sums of 32-bit words (each column
+// below 2^63, V below 2^20 p): a sum of canonical values added word by word with no carries.
+// The carries are resolved once, the quotient q = floor(V / p - 2^-20) comes from the top
+// words in double precision (its error is far below 2^-20, so q is floor(V / p) or one less),
+// and V - q p in [0, 2p) takes one conditional subtraction.
+template <class F>
+SC_HD Elem<F::L> reduce_word_columns(const uint64_t (&col)[F::L]) {
+  static_assert(F::BITS < 32 * F::L, "2p must fit the L words");
+  constexpr int L = F::L;
+  uint32_t w[L];
+  uint64_t c = 0;
+#pragma unroll
+  for (int k = 0; k < L; ++k) {
+    c += col[k];
+    w[k] = (uint32_t)c;
+    c >>= 32;
+  }
+  const uint32_t top = (uint32_t)c;  // V = w + top 2^(32L), top < 2^20
+  double pd = 0.0;                   // p / 2^(32(L-1)) as a double
+#pragma unroll
+  for (int k = 0; k < L; ++k) pd = pd * 0x1p-32 + (double)F::p(k);
+  double v = (double)top * 0x1p32 + (double)w[L - 1];
+  if (L > 1) v += (double)w[L > 1 ? L - 2 : 0] * 0x1p-32;
+  double q = floor(fmax(v / pd - 0x1p-20, 0.0));
+  const uint32_t qi = (uint32_t)q;
+  // x = V - q p: q p as L + 1 words, subtracted with borrow
+  Elem<L> x;
+  uint64_t m = 0;
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int k = 0; k < L; ++k) {
+    m += (uint64_t)qi * F::p(k);
+    const uint64_t d = (uint64_t)w[k] - (uint32_t)m - borrow;
+    x.v[k] = (uint32_t)d;
+    borrow = (uint32_t)(d >> 63);
+    m >>= 32;
+  }
+  // the top word of V - q p is top - m - borrow = 0 (x < 2p < 2^(32L)); x - p if x >= p
+  Elem<L> t;
+  const uint32_t b2 = sub_p<F>(t, x);
+#pragma unroll
+  for (int k = 0; k < L; ++k) x.v[k] = b2 ? x.v[k] : t.v[k];
+  return x;
 }
 
 template <class F>

@@ -783,10 +783,76 @@ __device__ __forceinline__ void finish_round(uint32_t* g, int npts, int round, i
 // Kernels
 // ---------------------------------------------------------------------------------------------
 
+// The round's claim g_{j-1}(r_{j-1}) into st->claim, for round j >= 1 of a plain proof: it
+// needs only the previous round's polynomial and challenge, so wave 0 of block 0 of the
+// round's pass (or point-parallel evaluation) kernel computes it before its own work, off the
+// path of the finalize, which reads it (a later launch on the same stream; with the pass
+// kernel's own last-block finish, the block-0 thread that stores it releases it with its
+// partial).
+template <class F>
+__device__ __forceinline__ void store_claim(ProofState* st, int round, int npts, const FieldConsts* __restrict__ fc) {
+  if (round > 0 && blockIdx.x == 0 && threadIdx.x < 64) {
+    const Elem<F::L> c = previous_claim<F>(st, round, npts, fc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < F::L; ++k) st->claim[k] = c.v[k];
+    }
+  }
+}
+
+// Sums over the blocks of a pass's partials (canonical values) for fields with 2p < 2^(32L):
+// the block's threads as PMAX * L (point, word) columns times S = BLOCK / (PMAX * L) segments of
+// the blocks, each adding its column's words in 64 bits with no carries (a wave reads whole
+// partials, 8 loads in flight per thread); the segments are added in LDS and one thread per
+// point reduces its L column sums once (reduce_word_columns). At 256 bits this takes the
+// finalize's partial sums and block reduction from 5.8-14 us to a few us: each add of the
+// element-wise form is a carry chain and a conditional subtraction, and the wave reduction
+// chained six of them per point.
+template <class F, int PMAX>
+__device__ __forceinline__ void sum_partials_cols(const uint32_t* __restrict__ partials, uint32_t nblocks, int npts,
+                                                  uint64_t* s_cols, uint32_t* s_g) {
+  constexpr int L = F::L;
+  constexpr int C = PMAX * L;
+  constexpr int S = BLOCK / C;
+  static_assert(C <= BLOCK, "columns of the block");
+  const int c = (int)threadIdx.x % C, seg = (int)threadIdx.x / C;  // threads past S * C idle
+  const int t = c / L, k = c % L;
+  uint64_t sum = 0;
+  if (t < npts && seg < S) {
+    for (uint32_t b0 = (uint32_t)seg; b0 < nblocks; b0 += 8u * S) {
+      uint32_t x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t b = b0 + (uint32_t)u * S;
+        x[u] = b < nblocks ? partials[((uint64_t)b * MAX_POINTS + t) * L + k] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) sum += x[u];
+    }
+  }
+  if (seg < S) s_cols[seg * C + c] = sum;
+  __syncthreads();
+  if ((int)threadIdx.x < C) {  // each thread adds up its own column only
+    uint64_t tot = 0;
+#pragma unroll
+    for (int q = 0; q < S; ++q) tot += s_cols[q * C + threadIdx.x];
+    s_cols[threadIdx.x] = tot;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < npts) {
+    uint64_t col[L];
+#pragma unroll
+    for (int w = 0; w < L; ++w) col[w] = s_cols[threadIdx.x * L + w];
+    const Elem<L> g = reduce_word_columns<F>(col);
+#pragma unroll
+    for (int w = 0; w < L; ++w) s_g[threadIdx.x * L + w] = g.v[w];
+  }
+  __syncthreads();
+}
+
 // Sums the per-block partials of a pass and finishes the round (g_j, r_j). Used by the
-// finalize kernel and by the last block of a pass kernel.
-// From round 1 on the last wave computes the round's claim (which needs only the previous
-// round's polynomial and challenge) into s_claim while the other waves sum the partials.
+// finalize kernel and by the last block of a pass kernel. From round 1 on the round's claim
+// is read from st->claim (store_claim).
 template <class F, int PMAX>
 __device__ __forceinline__ void reduce_partials_and_finish(const uint32_t* partials, uint32_t nblocks,
                                                            int npts, int round, int num_rounds, int mode,
@@ -795,26 +861,23 @@ __device__ __forceinline__ void reduce_partials_and_finish(const uint32_t* parti
                                                            uint32_t* s_g, uint32_t* s_msg, uint32_t* s_r,
                                                            uint32_t* s_claim, int scale_pow) {
   constexpr int L = F::L;
-  Elem<L> acc[PMAX];
-#pragma unroll
-  for (int t = 0; t < PMAX; ++t) acc[t] = zero<F>();
   const bool early = round > 0;
-  if (early && (threadIdx.x >> 6) == BLOCK / 64 - 1) {
-    const Elem<L> c = previous_claim<F>(st, round, npts, fc);
-    if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-      for (int k = 0; k < L; ++k) s_claim[k] = c.v[k];
-    }
+  if (early && threadIdx.x < L) s_claim[threadIdx.x] = st->claim[threadIdx.x];
+  if constexpr (F::BITS < 32 * L) {
+    __shared__ uint64_t s_cols[BLOCK];
+    sum_partials_cols<F, PMAX>(partials, nblocks, npts, s_cols, s_g);  // its barriers publish s_claim
   } else {
-    const uint32_t step = early ? BLOCK - 64 : BLOCK;
-    for (uint32_t b = threadIdx.x; b < nblocks; b += step) {
+    Elem<L> acc[PMAX];
+#pragma unroll
+    for (int t = 0; t < PMAX; ++t) acc[t] = zero<F>();
+    for (uint32_t b = threadIdx.x; b < nblocks; b += BLOCK) {
 #pragma unroll
       for (int t = 0; t < PMAX; ++t) {
         if (t < npts) acc[t] = add<F>(acc[t], load_elem<L>(partials + ((uint64_t)b * MAX_POINTS + t) * L));
       }
     }
+    block_reduce<F, PMAX, BLOCK>(acc, npts, s_red, s_g);  // its barriers publish s_claim too
   }
-  block_reduce<F, PMAX, BLOCK>(acc, npts, s_red, s_g);  // its barriers publish s_claim too
   if (scale_pow != 0) {  // the pass summed g / 32^scale_pow
     if (threadIdx.x < npts) {
       const Elem<L> g = mont_mul<F>(load_elem<L>(s_g + threadIdx.x * L), load_elem<L>(fc->pow32[scale_pow]));
@@ -882,6 +945,7 @@ pass_kernel(PassArgs a) {
   const int K = Shape<SID>::terms(cd);
   const int npts = Shape<SID>::npts(cd);
   const bool fold = a.fold != 0;
+  store_claim<F>(a.st, a.round, npts, a.fc);
 
   using W = Work<F>;
   typename W::Acc acc[PMAX];
@@ -1309,6 +1373,7 @@ __global__ void __launch_bounds__(BLOCK) wide_fold_kernel(PassArgs a) {
 template <class F, int TMAX, int G>
 __global__ void __launch_bounds__(BLOCK) wide_eval_kernel(PassArgs a) {
   __shared__ uint32_t s_red[BLOCK * F::L];
+  store_claim<F>(a.st, a.round, a.comp->npts, a.fc);
   const uint64_t pairs_here = wide_pairs_per_block(a.comp->npts, a.round);
   const TableMap sm = a.round == 0 ? strided_map(a.src_stride) : fold_map(a.src_n, a.comp->num_tables, a.chunk_shift);
   wide_eval_block<F, TMAX>(a.src, sm, a.round == 0, a.round, (uint64_t)blockIdx.x * pairs_here, a.pairs,

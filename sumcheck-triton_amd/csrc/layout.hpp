@@ -150,6 +150,9 @@ struct ProofState {
   uint32_t final_ready;
   uint32_t pad2[63];
   uint32_t final_vals[MAX_TABLES][MAX_LIMBS];
+  // Claim of the current round j >= 1 of a plain proof, g_{j-1}(r_{j-1}) (Montgomery), written
+  // by the round's pass or point-parallel evaluation kernel for its finalize.
+  uint32_t claim[MAX_LIMBS];
 };
 
 // Entries from one input table to the next in an input buffer the library allocates. Tables of

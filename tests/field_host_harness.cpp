@@ -202,6 +202,23 @@ extern "C" void r29_fold_tab_host(int mont, uint64_t n, const uint32_t* lo, cons
   }
 }
 
+// V mod p from L word-column sums per value (reduce_word_columns), 128- and 256-bit fields.
+extern "C" void word_columns_host(int bit_width, uint64_t n, const uint64_t* cols, uint32_t* out) {
+  for (uint64_t i = 0; i < n; ++i) {
+    if (bit_width == 128) {
+      uint64_t c[4];
+      for (int k = 0; k < 4; ++k) c[k] = cols[i * 4 + k];
+      const Elem<4> x = reduce_word_columns<FieldP128>(c);
+      for (int k = 0; k < 4; ++k) out[i * 4 + k] = x.v[k];
+    } else {
+      uint64_t c[8];
+      for (int k = 0; k < 8; ++k) c[k] = cols[i * 8 + k];
+      const Elem<8> x = reduce_word_columns<FieldP256>(c);
+      for (int k = 0; k < 8; ++k) out[i * 8 + k] = x.v[k];
+    }
+  }
+}
+
 extern "C" void interleave_host(uint64_t n, const uint64_t* w, uint32_t* even, uint32_t* odd, uint64_t* joined) {
   for (uint64_t i = 0; i < n; ++i) {
     even[i] = interleaved_half(w[i], 0);

@@ -37,6 +37,8 @@ def harness(tmp_path_factory):
     lib.gl_host_mul_sum.restype = None
     lib.r29_fold_tab_host.argtypes = [ctypes.c_int, ctypes.c_uint64, u32p, u32p, u32p, u32p]
     lib.r29_fold_tab_host.restype = None
+    lib.word_columns_host.argtypes = [ctypes.c_int, ctypes.c_uint64, u64p, u32p]
+    lib.word_columns_host.restype = None
     lib.interleave_host.argtypes = [ctypes.c_uint64, u64p, u32p, u32p, u64p]
     lib.interleave_host.restype = None
     lib.gl_host_accum.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32), u64p, u64p]
@@ -107,6 +109,15 @@ def harness(tmp_path_factory):
         return out
 
     op.fold_tab = fold_tab
+
+    def word_columns(bw, cols):
+        L = 4 if bw == 128 else 8
+        c = np.ascontiguousarray(cols, dtype=np.uint64).reshape(-1, L)
+        out = np.zeros(c.shape, dtype=np.uint32)
+        lib.word_columns_host(bw, c.shape[0], c.ctypes.data_as(u64p), out.ctypes.data_as(u32p))
+        return [sum(int(x) << (32 * k) for k, x in enumerate(row)) for row in out.tolist()]
+
+    op.word_columns = word_columns
 
     op.gl_mul_sum = gl_mul_sum
     op.gl_accum = gl_accum
@@ -266,6 +277,35 @@ def test_host_r29_fold_by_table(harness):
             assert all(x < (1 << 29) for x in row) and v < top, (r, a, b)
             assert v % p == 32 * (a + r * (b - a)) % p, (r, a, b)
     assert W > top
+
+
+@pytest.mark.parametrize("bw", [128, 256])
+def test_host_word_column_sums(harness, bw):
+    """Sums of canonical values added word by word into 64-bit columns (no carries), then
+    reduced once (reduce_word_columns, the block-partial sums of the 8-limb finalize): equal to
+    the sum mod p for 1 to 4096 values, all-(p - 1) and all-zero inputs, exact multiples of p,
+    and column sums whose carries reach the top word."""
+    f = get_field(bw)
+    p, L = f.modulus, f.limbs
+    rng = random.Random(bw + 7)
+    cases = []
+    for n in [1, 2, 3, 255, 256, 4096]:
+        cases.append([rng.randrange(p) for _ in range(n)])
+    cases += [[p - 1] * 4096, [0] * 17, [p - 1] * 3, [1] * 5]
+    # sums that land on or next to a multiple of p
+    for k in [1, 2, 1000, 4095]:
+        xs = [rng.randrange(p) for _ in range(k)]
+        s = sum(xs)
+        for target in (((s // p) + 1) * p, ((s // p) + 1) * p - 1, ((s // p) + 1) * p + 1):
+            extra = target - s
+            if 0 <= extra < p:
+                cases.append(xs + [extra])
+    cols = []
+    for xs in cases:
+        cols.append([sum((x >> (32 * k)) & 0xFFFFFFFF for x in xs) for k in range(L)])
+    got = harness.word_columns(bw, cols)
+    for xs, g in zip(cases, got):
+        assert g == sum(xs) % p, len(xs)
 
 
 def test_host_keccak_bit_interleaving(harness):
