@@ -8,8 +8,10 @@ Workload (one step = one whole proof, results copied to the host):
   under config.per_width.
 
 Prints ONE JSON line. `value` is the headline proof replayed as its captured hipGraph (one
-host launch per proof, the way a caller proving repeatedly runs it), K timed replays after
-the warmups. `roofline` is for the dominant kernel (the fused fold+eval pass kernel):
+host launch per proof), K timed replays after the warmups, pipelined the way a caller proving
+repeatedly runs them: each proof's results are copied by the device into a pinned host slot
+and read while the next proof runs (sc_proof_run_async). The same replays one at a time
+(replay, wait, read) are reported as `unpipelined_proofs_per_s`, with their p50/p99. `roofline` is for the dominant kernel (the fused fold+eval pass kernel):
 algorithmic bytes of every pass launch divided by their summed durations, both measured here
 with hipEvents recorded on the stream the kernels run on, over K eager proofs of the same
 instance (per-launch event records need eager launches); their rate is reported as
@@ -139,6 +141,22 @@ def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: floa
         ctx.sync()
         graph_wall = time.perf_counter() - g0
         graph_ms = graph_wall * 1000.0 / steps
+        # the same replays pipelined as a caller proving repeatedly runs them: proof k + 1 is
+        # enqueued before proof k's results (copied by the device into a pinned slot) are read,
+        # so the host's work overlaps the device's
+        prev = pr.run_async("sha3")
+        pr.slot_result(prev)
+        ctx.sync()
+        p0 = time.perf_counter()
+        prev = None
+        for _ in range(steps):
+            slot = pr.run_async("sha3")
+            if prev is not None:
+                pr.slot_result(prev)
+            prev = slot
+        pr.slot_result(prev)
+        ctx.sync()
+        pipe_wall = time.perf_counter() - p0
     lat.sort()
     glat.sort()
     achieved = pass_bytes_sum / (pass_ms_sum * 1e6) if pass_ms_sum > 0 else 0.0
@@ -166,6 +184,8 @@ def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: floa
         "graph_proofs_per_s": steps / graph_wall,
         "graph_p50_ms": statistics.median(glat),
         "graph_p99_ms": glat[min(len(glat) - 1, int(round(0.99 * (len(glat) - 1))))],
+        "pipelined_ms_per_proof": pipe_wall * 1000.0 / steps,
+        "pipelined_proofs_per_s": steps / pipe_wall,
         "wall_s": wall,
     }
 
@@ -491,12 +511,12 @@ def main() -> int:
         cpu = cpu_baseline()
     line = {
         "metric": METRIC,
-        "value": round(head["graph_proofs_per_s"], 3),
+        "value": round(head["pipelined_proofs_per_s"], 3),
         "unit": "proofs/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(head["graph_ms_per_proof"], 4),
+        "ms_per_step": round(head["pipelined_ms_per_proof"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -509,9 +529,12 @@ def main() -> int:
             "num_vars": 26,
             "tables": 5,
             "parallelism": "single GPU",
-            "step": "one whole proof replayed as its captured hipGraph, results copied to the host",
+            "step": "one whole proof replayed as its captured hipGraph, its results copied to pinned "
+                    "host memory and read by the host while the next proof runs",
             "p50_ms": round(head["graph_p50_ms"], 4),
             "p99_ms": round(head["graph_p99_ms"], 4),
+            "latency": "p50/p99 of one proof run alone: replay, wait, results read",
+            "unpipelined_proofs_per_s": round(head["graph_proofs_per_s"], 3),
             "eager_proofs_per_s": round(head["proofs_per_s"], 3),
             "per_width": {str(bw): _rounded(r) for bw, r in per.items()},
             "goldilocks_abc_nv16": _rounded(cfg1),

@@ -131,6 +131,17 @@ int sc_proof_wait(sc_proof* proof, float* ms);
 int sc_proof_result(sc_proof* proof, uint32_t* round_polys, uint32_t* challenges,
                     uint64_t* checksum);
 
+/* Repeated proving with the host's work overlapped with the device's: sc_proof_run_async
+ * enqueues the proof as sc_proof_run does and, after it on the same stream, a copy of its
+ * round polynomials and challenges into one of two pinned host slots the proof owns; *slot
+ * names it. sc_proof_slot_result waits for that copy only (not for later runs) and returns the
+ * run's results as sc_proof_result does. A slot is reused by the run after next: read a run's
+ * results before enqueuing two more. No counterpart in the reference, whose driver waits for
+ * each run (field_sumcheck.cu:1012-1088). */
+int sc_proof_run_async(sc_proof* proof, int32_t challenge_mode, const uint32_t* explicit_r, int32_t* slot);
+int sc_proof_slot_result(sc_proof* proof, int32_t slot, uint32_t* round_polys, uint32_t* challenges,
+                         uint64_t* checksum);
+
 /* Copies the prepared input tables (Montgomery, element-major) to host memory. */
 int sc_proof_tables(sc_proof* proof, uint32_t* out);
 

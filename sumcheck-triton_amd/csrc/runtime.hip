@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstdint>
 #include <cstdio>
 #include <deque>
@@ -294,6 +295,11 @@ struct sc_proof {
   bool profiled_last = false;
   hipEvent_t prof_ev[2 * (MAX_ROUNDS + 1)] = {};
   hipEvent_t prof_fin[MAX_ROUNDS] = {};  // after round j's finalize
+  // sc_proof_run_async: two pinned host slots for the challenges and round polynomials of a
+  // run (ProofState from r up to the proof's last polynomial row), each with its copy's event
+  uint32_t* h_slot[2] = {nullptr, nullptr};
+  hipEvent_t ev_slot[2] = {nullptr, nullptr};
+  uint64_t async_runs = 0;
 };
 
 namespace {
@@ -610,6 +616,10 @@ void free_proof(sc_proof* p) {
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : p->prof_fin)
     if (e) (void)hipEventDestroy(e);
+  for (int k = 0; k < 2; ++k) {
+    if (p->h_slot[k]) (void)hipHostFree(p->h_slot[k]);
+    if (p->ev_slot[k]) (void)hipEventDestroy(p->ev_slot[k]);
+  }
   delete p;
 }
 
@@ -941,6 +951,37 @@ int sc_proof_wait(sc_proof* p, float* ms) {
   return SC_OK;
 }
 
+namespace {
+
+// Bytes of ProofState a result slot holds: r (all rounds) and polys up to the proof's last
+// round, one contiguous range.
+uint64_t slot_bytes(const sc_proof* p) {
+  return offsetof(ProofState, polys) - offsetof(ProofState, r) +
+         (uint64_t)p->num_vars * sizeof(((ProofState*)nullptr)->polys[0]);
+}
+
+// The challenges and round polynomials of a ProofState image (its r and polys arrays) in the
+// layout of sc_proof_result, with the checksum.
+void decode_result(const sc_proof* p, const uint32_t (*r)[MAX_LIMBS], const uint32_t (*polys)[MAX_POINTS][MAX_LIMBS],
+                   uint32_t* round_polys, uint32_t* challenges, uint64_t* checksum) {
+  uint64_t sum = 0;
+  for (int j = 0; j < p->num_vars; ++j) {
+    for (int t = 0; t < p->npts; ++t) {
+      for (int i = 0; i < p->L; ++i) {
+        const uint32_t w = polys[j][t][i];
+        sum += (uint64_t)(i + 1) * w;
+        if (round_polys) round_polys[((size_t)j * p->npts + t) * p->L + i] = w;
+      }
+    }
+    if (challenges && j + 1 < p->num_vars) {
+      for (int i = 0; i < p->L; ++i) challenges[(size_t)j * p->L + i] = r[j][i];
+    }
+  }
+  if (checksum) *checksum = sum;
+}
+
+}  // namespace
+
 int sc_proof_result(sc_proof* p, uint32_t* round_polys, uint32_t* challenges, uint64_t* checksum) {
   if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
   SC_HIP(hipSetDevice(p->ctx->device));
@@ -948,20 +989,42 @@ int sc_proof_result(sc_proof* p, uint32_t* round_polys, uint32_t* challenges, ui
   std::vector<ProofState> host(1);
   SC_HIP(hipMemcpy(host.data(), p->d_state, sizeof(ProofState), hipMemcpyDeviceToHost));
   const ProofState& st = host[0];
-  uint64_t sum = 0;
-  for (int j = 0; j < p->num_vars; ++j) {
-    for (int t = 0; t < p->npts; ++t) {
-      for (int i = 0; i < p->L; ++i) {
-        const uint32_t w = st.polys[j][t][i];
-        sum += (uint64_t)(i + 1) * w;
-        if (round_polys) round_polys[((size_t)j * p->npts + t) * p->L + i] = w;
-      }
-    }
-    if (challenges && j + 1 < p->num_vars) {
-      for (int i = 0; i < p->L; ++i) challenges[(size_t)j * p->L + i] = st.r[j][i];
+  decode_result(p, st.r, st.polys, round_polys, challenges, checksum);
+  return SC_OK;
+}
+
+int sc_proof_run_async(sc_proof* p, int32_t mode, const uint32_t* explicit_r, int32_t* slot) {
+  if (p == nullptr || slot == nullptr) return fail(SC_ERR_ARG, "proof and slot must not be NULL");
+  SC_HIP(hipSetDevice(p->ctx->device));
+  if (p->h_slot[0] == nullptr) {
+    for (int k = 0; k < 2; ++k) {
+      SC_HIP(hipHostMalloc((void**)&p->h_slot[k], slot_bytes(p), hipHostMallocDefault));
+      SC_HIP(hipEventCreateWithFlags(&p->ev_slot[k], hipEventDisableTiming));
     }
   }
-  if (checksum) *checksum = sum;
+  int rc = sc_proof_run(p, mode, explicit_r);
+  if (rc != SC_OK) return rc;
+  const int k = (int)(p->async_runs & 1u);
+  hipStream_t s = p->ctx->stream;
+  SC_HIP(hipMemcpyAsync(p->h_slot[k], &p->d_state->r[0][0], slot_bytes(p), hipMemcpyDeviceToHost, s));
+  SC_HIP(hipEventRecord(p->ev_slot[k], s));
+  ++p->async_runs;
+  *slot = k;
+  return SC_OK;
+}
+
+int sc_proof_slot_result(sc_proof* p, int32_t slot, uint32_t* round_polys, uint32_t* challenges,
+                         uint64_t* checksum) {
+  if (p == nullptr) return fail(SC_ERR_ARG, "proof must not be NULL");
+  if (slot < 0 || slot > 1 || p->h_slot[slot] == nullptr)
+    return fail(SC_ERR_ARG, "slot must be one returned by sc_proof_run_async");
+  SC_HIP(hipSetDevice(p->ctx->device));
+  SC_HIP(hipEventSynchronize(p->ev_slot[slot]));
+  const uint8_t* base = reinterpret_cast<const uint8_t*>(p->h_slot[slot]);
+  const auto* r = reinterpret_cast<const uint32_t(*)[MAX_LIMBS]>(base);
+  const auto* polys = reinterpret_cast<const uint32_t(*)[MAX_POINTS][MAX_LIMBS]>(
+      base + offsetof(ProofState, polys) - offsetof(ProofState, r));
+  decode_result(p, r, polys, round_polys, challenges, checksum);
   return SC_OK;
 }
 

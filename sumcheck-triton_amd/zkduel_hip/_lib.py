@@ -77,6 +77,8 @@ SIGNATURES = [
     ("sc_proof_capture", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, u32p]),
     ("sc_proof_wait", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
     ("sc_proof_result", ctypes.c_int, [ctypes.c_void_p, u32p, u32p, u64p]),
+    ("sc_proof_run_async", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, u32p, ctypes.POINTER(ctypes.c_int32)]),
+    ("sc_proof_slot_result", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, u32p, u32p, u64p]),
     ("sc_proof_tables", ctypes.c_int, [ctypes.c_void_p, u32p]),
     ("sc_proof_info", ctypes.c_int, [ctypes.c_void_p, i32p, i32p, u64p, ctypes.POINTER(ctypes.c_int64)]),
     ("sc_proof_use_compiled", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),

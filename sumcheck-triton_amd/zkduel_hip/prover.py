@@ -269,6 +269,25 @@ class Proof:
                                             ctypes.byref(cs)))
         return polys, chal[: self.num_vars - 1], int(cs.value)
 
+    def run_async(self, challenge_mode: str = "sha3", challenges=None) -> int:
+        """Enqueues the proof and a copy of its results into a pinned host slot; returns the
+        slot for slot_result. A slot is reused two runs later (read it before then)."""
+        mode = _lib.CHALLENGE_MODES[challenge_mode]
+        r = self._explicit(mode, challenges)
+        slot = ctypes.c_int32()
+        _lib.check(self.lib.sc_proof_run_async(self.handle, mode, _lib.as_u32p(r), ctypes.byref(slot)))
+        return int(slot.value)
+
+    def slot_result(self, slot: int) -> tuple[np.ndarray, np.ndarray, int]:
+        """result() of the run that filled slot, waiting for that run's copy only."""
+        L = self.field.limbs
+        polys = np.zeros((self.num_vars, self.npts, L), dtype=np.uint32)
+        chal = np.zeros((max(self.num_vars - 1, 1), L), dtype=np.uint32)
+        cs = ctypes.c_uint64()
+        _lib.check(self.lib.sc_proof_slot_result(self.handle, int(slot), _lib.as_u32p(polys), _lib.as_u32p(chal),
+                                                 ctypes.byref(cs)))
+        return polys, chal[: self.num_vars - 1], int(cs.value)
+
     def tables(self) -> np.ndarray:
         out = np.empty((self.comp.num_tables, 1 << self.num_vars, self.field.limbs), dtype=np.uint32)
         _lib.check(self.lib.sc_proof_tables(self.handle, _lib.as_u32p(out)))

@@ -391,6 +391,39 @@ def test_schedule_and_graph_replays_are_identical(ctx, bw):
             pr.wait()
 
 
+@pytest.mark.parametrize("bw", [64, 256])
+def test_pipelined_runs_return_each_runs_results(ctx, bw):
+    """sc_proof_run_async / sc_proof_slot_result as a repeated-proving loop runs them: each run
+    is enqueued before the previous run's results are read, and consecutive runs differ (SHA3
+    and fixed challenges alternate), so a slot read too early or overwritten shows. Eager runs,
+    then with the SHA3 runs replayed from a captured graph."""
+    comp = Composition.from_spec(get_spec("poly_abc_plus_de"))
+    with Proof(ctx, bw, comp, 14, input="uniform", seed=5) as pr:
+        want = {}
+        for mode in ("sha3", "fixed"):
+            pr.run(mode)
+            pr.wait()
+            want[mode] = pr.result()
+        assert not np.array_equal(want["sha3"][0], want["fixed"][0])
+        modes = ["sha3", "fixed", "fixed", "sha3", "sha3", "fixed", "sha3", "fixed"]
+        for graph in (False, True):
+            if graph:
+                pr.capture("sha3")
+            say("pipelined", bw, "graph" if graph else "eager")
+            prev = None
+            for m in modes:
+                slot = pr.run_async(m)
+                if prev is not None:
+                    got = pr.slot_result(prev[1])
+                    exp = want[prev[0]]
+                    assert np.array_equal(got[0], exp[0]) and np.array_equal(got[1], exp[1]) and got[2] == exp[2]
+                prev = (m, slot)
+            got = pr.slot_result(prev[1])
+            assert np.array_equal(got[0], want[prev[0]][0]) and got[2] == want[prev[0]][2]
+        with pytest.raises(ValueError):
+            pr.slot_result(2)
+
+
 def _sweep_cases():
     keep = []
     for g in SWEEP:
