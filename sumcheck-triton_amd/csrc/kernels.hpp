@@ -339,6 +339,66 @@ struct Shape {
   }
 };
 
+// Compiled compositions whose first term is a*b*c with a unit coefficient and whose degree is
+// 3 (a*b*c, a*b*c + d*e), at 255 bits: round 0 (points 0, 1, 2 and the leading coefficient)
+// by eval_round0_abc.
+template <class F, int SID>
+constexpr bool round0_abc_form() {
+  if constexpr (!std::is_same_v<F, FieldP256> || SID == 0) {
+    return false;
+  } else {
+    constexpr StaticComp C = STATIC_COMPS[SID];
+    return static_degree(SID) == 3 && C.lens[0] == 3 && C.kinds[0] == COEFF_ONE &&
+           (C.num_terms == 1 || (C.num_terms == 2 && C.kinds[1] == COEFF_ONE && C.lens[1] == 2));
+  }
+}
+
+// Round 0 of round0_abc_form compositions. a(t) b(t) is quadratic in t, so
+//   a(2) b(2) = 2 a(1) b(1) - a(0) b(0) + 2 (a(1) - a(0)) (b(1) - b(0)),
+// and that last product is the one the leading coefficient (a b c over the differences) needs
+// anyway: per pair one product and the interpolation of a and b at t = 2 fewer than the
+// general order (round 0 issues VALU at the mad probe's rate, so instructions are its time).
+template <class F, int TMAX, int PMAX, int SID, class A>
+__device__ __forceinline__ void eval_round0_abc(typename Work<F>::V (&cur)[TMAX], typename Work<F>::V (&oth)[TMAX],
+                                                A (&acc)[PMAX]) {
+  using W = Work<F>;
+  using V = typename W::V;
+  constexpr StaticComp C = STATIC_COMPS[SID];
+  constexpr int ia = C.vars[0][0], ib = C.vars[0][1], ic = C.vars[0][2];
+  static_assert(PMAX >= 4, "points 0, 1, 2 and the leading coefficient");
+  auto finish = [&](A& ac, const V& ab) {  // ac += ab c (+ d e)
+    if constexpr (C.num_terms == 2) {
+      ac.add(W::mul_sum(ab, cur[ic], cur[C.vars[1][0]], cur[C.vars[1][1]]));
+    } else {
+      acc_add_mul<F>(ac, ab, cur[ic]);
+    }
+  };
+  const V ab0 = W::mul(cur[ia], cur[ib]);
+  finish(acc[0], ab0);
+  static_for<C.num_tables>([&](auto vi) {
+    constexpr int v = decltype(vi)::value;
+    const V o = oth[v];
+    oth[v] = W::diff(o, cur[v]);
+    cur[v] = o;
+  });
+  const V ab1 = W::mul(cur[ia], cur[ib]);
+  finish(acc[1], ab1);
+  const V dadb = W::mul(oth[ia], oth[ib]);
+  acc_add_mul<F>(acc[3], dadb, oth[ic]);
+  const V ab2 = r29_two_x_plus_two_y_minus_z(ab1, dadb, ab0);
+  static_for<C.num_tables>([&](auto vi) {
+    constexpr int v = decltype(vi)::value;
+    if constexpr (v != ia && v != ib) {
+      if constexpr (static_lazy_step(SID, v, 2)) {
+        cur[v] = W::step_lazy(cur[v], oth[v]);
+      } else {
+        cur[v] = W::step(cur[v], oth[v]);
+      }
+    }
+  });
+  finish(acc[2], ab2);
+}
+
 // cur holds f[2i] and oth holds f[2i+1] of every table on entry. Accumulates
 // C(f[2i] + t (f[2i+1] - f[2i])) into acc[t] for t < d = npts - 1, with t >= 2 reached by
 // repeated addition of the difference (reference.py:89-99; those sums may stay unreduced
@@ -350,6 +410,12 @@ __device__ __forceinline__ void eval_points(typename Work<F>::V (&cur)[TMAX], ty
                                             A (&acc)[PMAX], const CompDesc* __restrict__ cd,
                                             int T, int K, int npts, bool skip1) {
   using W = Work<F>;
+  if constexpr (LEAD && round0_abc_form<F, SID>()) {
+    if (!skip1) {
+      eval_round0_abc<F, TMAX, PMAX, SID>(cur, oth, acc);
+      return;
+    }
+  }
   constexpr bool X = SID != 0 || TMAX <= 8;  // expand every loop unless the table count is large
   const int d = LEAD ? npts - 1 : npts;  // the points t < d are evaluated
   compose<F, SID, TMAX>(acc[0], cur, cd, K);

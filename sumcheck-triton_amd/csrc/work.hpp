@@ -517,6 +517,16 @@ SC_HD R29 r29_fold_finish(uint64_t (&c)[9]) {
   return x;
 }
 
+// 2 x + 2 y - z for products x, y, z (limbs 0..7 below 2^29, values below 2p; the round-0
+// identity of the compiled a*b*c compositions): value below 7.5 p, normalized, so it can be the
+// first factor of a product.
+SC_HD R29 r29_two_x_plus_two_y_minus_z(const R29& x, const R29& y, const R29& z) {
+  R29 s;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) s.l[i] = 2u * x.l[i] + 2u * y.l[i] + r29::PAD(3, i) - z.l[i];
+  return r29_normalize(s);
+}
+
 // Eight 32-bit words W -> limbs of W * 2^S (S = 0: fold buffers; S = 5: Montgomery input,
 // turning X = x 2^256 into 32 X = x 2^261).
 template <int S>

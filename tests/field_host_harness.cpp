@@ -49,6 +49,8 @@ extern "C" void field_host_op(int bit_width, int op, uint64_t n, const uint32_t*
 //   op 9: a read scaled (from_mont_scaled: the residue times 2^-5; else from_mont)
 //   op 10: a x3 + (a x2 + c x2) with x2, x3 = x(2), x(3) left unnormalized (step_lazy), the
 //          largest second operands the compiled compositions pass to mul and mul_sum
+//   op 11: a(2) b(2) b with a(0) = a, a(1) = b, b(0) = c, b(1) = a; at 255 bits through the
+//          round-0 identity 2 a1 b1 - a0 b0 + 2 (a1 - a0)(b1 - b0) (r29_two_x_plus_two_y_minus_z)
 template <class F>
 static void run_work(int op, uint64_t n, const uint32_t* a, const uint32_t* b, const uint32_t* c, uint32_t* out) {
   using W = Work<F>;
@@ -103,6 +105,17 @@ static void run_work(int op, uint64_t n, const uint32_t* a, const uint32_t* b, c
       }
       acc.add(W::mul(x, x3));
       acc.add(W::mul_sum(x, x2, z, x2));
+    } else if (op == 11) {
+      // a(2) b(2) c from a(0) = a, a(1) = b, b(0) = c, b(1) = a, c = b as round 0 of the
+      // compiled a*b*c compositions forms it at 255 bits: 2 a1 b1 - a0 b0 + 2 (a1 - a0)(b1 - b0)
+      if constexpr (W::kFoldTable) {
+        const typename W::V ab0 = W::mul(x, z), ab1 = W::mul(y, x);
+        const typename W::V dadb = W::mul(W::diff(y, x), W::diff(x, z));
+        acc.add(W::mul(r29_two_x_plus_two_y_minus_z(ab1, dadb, ab0), y));
+      } else {
+        const typename W::V a2 = W::step(y, W::diff(y, x)), b2 = W::step(x, W::diff(x, z));
+        acc.add(W::mul(W::mul(a2, b2), y));
+      }
     } else if (op == 6 || op == 7) {
       // a single-factor term with coefficient -1 is the first term a thread adds: the
       // subtraction must hold for the largest interpolated value on its own

@@ -188,6 +188,9 @@ def test_host_working_form(harness, bw):
     # products whose second operands are interpolated values left unnormalized
     x2s = [(x + 2 * (y - x)) % p for x, y in zip(a, b)]
     x3s = [(x + 3 * (y - x)) % p for x, y in zip(a, b)]
+    # op 11: a(2) = 2b - a, b(2) = 2a - c, times b (Montgomery products)
+    assert harness.work(bw, 11, a, b, c) == [mont(mont((2 * y - x) % p, (2 * x - z) % p), y)
+                                             for x, y, z in zip(a, b, c)]
     assert harness.work(bw, 10, a, b, c) == [(mont(x, u3) + mont(x, u2) + mont(z, u2)) % p
                                              for x, z, u2, u3 in zip(a, c, x2s, x3s)]
 
