@@ -849,21 +849,32 @@ __device__ __forceinline__ void finish_round(uint32_t* g, int npts, int round, i
 // Kernels
 // ---------------------------------------------------------------------------------------------
 
-// The round's claim g_{j-1}(r_{j-1}) into st->claim, for round j >= 1 of a plain proof: it
-// needs only the previous round's polynomial and challenge, so wave 0 of block 0 of the
-// round's pass (or point-parallel evaluation) kernel computes it before its own work, off the
-// path of the finalize, which reads it (a later launch on the same stream; with the pass
-// kernel's own last-block finish, the block-0 thread that stores it releases it with its
-// partial).
+// The round's claim g_{j-1}(r_{j-1}) into st->claim, for round j >= 1 of a plain proof. It
+// needs only the previous round's polynomial and challenge, so the round's pass (or
+// point-parallel evaluation) kernel computes it, off the path of the finalize, which reads it
+// (a later launch on the same stream): wave 0 of one extra block at the end of the grid
+// (claim_block) that does nothing else, so no evaluating block is held up by it (as block 0
+// computing it first would hold up a kernel of equal blocks by its length); with the pass
+// kernel's own last-block finish there is no extra block, and block 0 computes it first (the
+// thread that stores it releases it with its partial). Returns true for the extra block,
+// which then leaves.
 template <class F>
-__device__ __forceinline__ void store_claim(ProofState* st, int round, int npts, const FieldConsts* __restrict__ fc) {
-  if (round > 0 && blockIdx.x == 0 && threadIdx.x < 64) {
+__device__ __forceinline__ bool store_claim(ProofState* st, int round, int npts, const FieldConsts* __restrict__ fc,
+                                            int claim_block) {
+  const bool extra = (int)blockIdx.x == claim_block;
+  if (round > 0 && (extra || (claim_block < 0 && blockIdx.x == 0)) && threadIdx.x < 64) {
     const Elem<F::L> c = previous_claim<F>(st, round, npts, fc);
     if (threadIdx.x == 0) {
 #pragma unroll
       for (int k = 0; k < F::L; ++k) st->claim[k] = c.v[k];
     }
   }
+  return extra;
+}
+
+// Blocks of a pass or point-parallel evaluation grid that evaluate (all but the claim block).
+__device__ __forceinline__ uint32_t evaluating_blocks(int claim_block) {
+  return claim_block >= 0 ? (uint32_t)claim_block : gridDim.x;
 }
 
 // Sums over the blocks of a pass's partials (canonical values) for fields with 2p < 2^(32L):
@@ -1011,7 +1022,7 @@ pass_kernel(PassArgs a) {
   const int K = Shape<SID>::terms(cd);
   const int npts = Shape<SID>::npts(cd);
   const bool fold = a.fold != 0;
-  store_claim<F>(a.st, a.round, npts, a.fc);
+  if (store_claim<F>(a.st, a.round, npts, a.fc, a.claim_block)) return;
 
   using W = Work<F>;
   typename W::Acc acc[PMAX];
@@ -1022,7 +1033,7 @@ pass_kernel(PassArgs a) {
   const TableMap sm = source_map(a.round, a.src_stride, a.src_n, T, a.chunk_shift);
   const TableMap dm = fold_map(a.src_n >> 1, T, a.chunk_shift);
 
-  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  const uint64_t stride = (uint64_t)evaluating_blocks(a.claim_block) * BLOCK;
   // The 2^127 - 1 field loads all tables of a pair first, in one loop per round kind
   // (load_pair_fixed; a*b*c + d*e num_vars 24 on MI355X: 1.69 -> 1.63 ms, same box). The
   // 255-bit field loads one table ahead of the one it converts (load_pair_pipelined): a whole
@@ -1439,11 +1450,11 @@ __global__ void __launch_bounds__(BLOCK) wide_fold_kernel(PassArgs a) {
 template <class F, int TMAX, int G>
 __global__ void __launch_bounds__(BLOCK) wide_eval_kernel(PassArgs a) {
   __shared__ uint32_t s_red[BLOCK * F::L];
-  store_claim<F>(a.st, a.round, a.comp->npts, a.fc);
+  if (store_claim<F>(a.st, a.round, a.comp->npts, a.fc, a.claim_block)) return;
   const uint64_t pairs_here = wide_pairs_per_block(a.comp->npts, a.round);
   const TableMap sm = a.round == 0 ? strided_map(a.src_stride) : fold_map(a.src_n, a.comp->num_tables, a.chunk_shift);
   wide_eval_block<F, TMAX>(a.src, sm, a.round == 0, a.round, (uint64_t)blockIdx.x * pairs_here, a.pairs,
-                           (uint64_t)gridDim.x * pairs_here, a.comp,
+                           (uint64_t)evaluating_blocks(a.claim_block) * pairs_here, a.comp,
                            a.partials + (uint64_t)blockIdx.x * MAX_POINTS * F::L, s_red);
 }
 

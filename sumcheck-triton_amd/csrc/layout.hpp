@@ -189,6 +189,10 @@ struct PassArgs {
   ProofState* st;
   uint32_t* partials;         // [gridDim][MAX_POINTS][L]
   const uint32_t* explicit_r; // [num_rounds-1][L] canonical, CHALLENGE_EXPLICIT only
+  // Block that only computes the round's claim into st->claim (the grid's last, one past the
+  // blocks that evaluate), or -1: then block 0 computes it before its own work.
+  int32_t claim_block = -1;
+  int32_t pad1 = 0;
 };
 
 // Rounds first_round..num_rounds-1 in one workgroup. Round j >= 1 reads buf(j-1) and writes

@@ -504,9 +504,12 @@ int enqueue_rounds(sc_proof* p, int mode) {
       e.fused_finalize = 0;
       const uint64_t pairs_per_block = wide_pairs_per_block(p->npts, j);
       g = std::min<uint64_t>((a.pairs + pairs_per_block - 1) / pairs_per_block, (uint64_t)MAX_PASS_BLOCKS);
-      SC_HIP(p->ln->wide_eval(s, p->tmax, p->pmax, e, (unsigned)g));
+      // from round 1 one more block computes the round's claim for the finalize
+      e.claim_block = j > 0 ? (int32_t)g : -1;
+      SC_HIP(p->ln->wide_eval(s, p->tmax, p->pmax, e, (unsigned)(g + (j > 0 ? 1 : 0))));
     } else {
-      SC_HIP(p->ln->pass(s, p->sid, p->tmax, p->pmax, a, (unsigned)g));
+      a.claim_block = (j > 0 && !p->fused_finalize) ? (int32_t)g : -1;
+      SC_HIP(p->ln->pass(s, p->sid, p->tmax, p->pmax, a, (unsigned)(g + (a.claim_block >= 0 ? 1 : 0))));
     }
     if (p->profiling) SC_HIP(hipEventRecord(p->prof_ev[2 * j + 1], s));
     if (p->fused_finalize && !wide) {
