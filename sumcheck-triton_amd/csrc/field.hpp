@@ -123,10 +123,10 @@ SC_HD Elem<F::L> add_limbs(const Elem<F::L>& a, const Elem<F::L>& b) {
 // below 2^63, V below 2^20 p): a sum of canonical values added word by word with no carries.
 // The carries are resolved once, the quotient q = floor(V / p - 2^-20) comes from the top
 // words in double precision (its error is far below 2^-20, so q is floor(V / p) or one less),
-// and V - q p in [0, 2p) takes one conditional subtraction.
+// and V - q p in [0, 2p) (L words and a top bit: 2p can exceed 2^(32L)) takes one conditional
+// subtraction.
 template <class F>
 SC_HD Elem<F::L> reduce_word_columns(const uint64_t (&col)[F::L]) {
-  static_assert(F::BITS < 32 * F::L, "2p must fit the L words");
   constexpr int L = F::L;
   uint32_t w[L];
   uint64_t c = 0;
@@ -136,7 +136,7 @@ SC_HD Elem<F::L> reduce_word_columns(const uint64_t (&col)[F::L]) {
     w[k] = (uint32_t)c;
     c >>= 32;
   }
-  const uint32_t top = (uint32_t)c;  // V = w + top 2^(32L), top < 2^20
+  const uint32_t top = (uint32_t)c;  // V = w + top 2^(32L), top < 2^21
   double pd = 0.0;                   // p / 2^(32(L-1)) as a double
 #pragma unroll
   for (int k = 0; k < L; ++k) pd = pd * 0x1p-32 + (double)F::p(k);
@@ -156,11 +156,12 @@ SC_HD Elem<F::L> reduce_word_columns(const uint64_t (&col)[F::L]) {
     borrow = (uint32_t)(d >> 63);
     m >>= 32;
   }
-  // the top word of V - q p is top - m - borrow = 0 (x < 2p < 2^(32L)); x - p if x >= p
+  const uint32_t xtop = top - (uint32_t)m - borrow;  // 0 or 1: x + xtop 2^(32L) < 2p
   Elem<L> t;
-  const uint32_t b2 = sub_p<F>(t, x);
+  const uint32_t b2 = sub_p<F>(t, x);  // t = x - p (mod 2^(32L)), the value when x >= p
+  const bool ge = xtop != 0u || b2 == 0u;
 #pragma unroll
-  for (int k = 0; k < L; ++k) x.v[k] = b2 ? x.v[k] : t.v[k];
+  for (int k = 0; k < L; ++k) x.v[k] = ge ? t.v[k] : x.v[k];
   return x;
 }
 

@@ -877,43 +877,58 @@ __device__ __forceinline__ uint32_t evaluating_blocks(int claim_block) {
   return claim_block >= 0 ? (uint32_t)claim_block : gridDim.x;
 }
 
-// Sums over the blocks of a pass's partials (canonical values) for fields with 2p < 2^(32L):
-// the block's threads as PMAX * L (point, word) columns times S = BLOCK / (PMAX * L) segments of
-// the blocks, each adding its column's words in 64 bits with no carries (a wave reads whole
-// partials, 8 loads in flight per thread); the segments are added in LDS and one thread per
-// point reduces its L column sums once (reduce_word_columns). At 256 bits this takes the
-// finalize's partial sums and block reduction from 5.8-14 us to a few us: each add of the
-// element-wise form is a carry chain and a conditional subtraction, and the wave reduction
-// chained six of them per point.
+// Sums over the blocks of a pass's partials (canonical values): each partial row (the first 4 or
+// 8 point slots, L words each, of one block) is read by RW / 4 threads, 16 bytes each, so the
+// block's threads cover S = 4 BLOCK / RW rows at a time with every load of a thread in flight together;
+// each thread adds its four words across its rows in 64-bit columns (no carries; point slots
+// past npts hold stale words and are summed but never read). The per-thread column sums are
+// added in LDS, four per thread and then one column per thread, and one thread per point
+// reduces its L column sums once (reduce_word_columns). At 256 bits the element-wise form
+// took the finalize's partial sums and block reduction 5.8-14 us (each add is a carry chain and
+// a conditional subtraction, and the wave reduction chained six of them per point).
 template <class F, int PMAX>
 __device__ __forceinline__ void sum_partials_cols(const uint32_t* __restrict__ partials, uint32_t nblocks, int npts,
                                                   uint64_t* s_cols, uint32_t* s_g) {
   constexpr int L = F::L;
-  constexpr int C = PMAX * L;
-  constexpr int S = BLOCK / C;
-  static_assert(C <= BLOCK, "columns of the block");
-  const int c = (int)threadIdx.x % C, seg = (int)threadIdx.x / C;  // threads past S * C idle
-  const int t = c / L, k = c % L;
-  uint64_t sum = 0;
-  if (t < npts && seg < S) {
-    for (uint32_t b0 = (uint32_t)seg; b0 < nblocks; b0 += 8u * S) {
-      uint32_t x[8];
+  constexpr int RW = (PMAX <= 4 ? 4 : 8) * L;  // words of a row read (point slots 0..3 or 0..7)
+  constexpr int TPR = RW / 4;                  // threads per row
+  constexpr int S = BLOCK / TPR;
+  static_assert(RW % 4 == 0 && BLOCK % TPR == 0, "16-byte loads over whole rows");
+  const int q = (int)threadIdx.x % TPR, seg = (int)threadIdx.x / TPR;
+  uint64_t sum[4] = {0, 0, 0, 0};
+  for (uint32_t b0 = (uint32_t)seg; b0 < nblocks; b0 += 8u * S) {
+    u32x4_t x[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const uint32_t b = b0 + (uint32_t)u * S;
-        x[u] = b < nblocks ? partials[((uint64_t)b * MAX_POINTS + t) * L + k] : 0u;
-      }
+    for (int u = 0; u < 8; ++u) {
+      const uint32_t b = b0 + (uint32_t)u * S;
+      x[u] = b < nblocks ? reinterpret_cast<const u32x4_t*>(partials + (uint64_t)b * MAX_POINTS * L)[q]
+                         : u32x4_t{0u, 0u, 0u, 0u};
+    }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) sum += x[u];
+    for (int u = 0; u < 8; ++u) {
+      sum[0] += x[u].x;
+      sum[1] += x[u].y;
+      sum[2] += x[u].z;
+      sum[3] += x[u].w;
     }
   }
-  if (seg < S) s_cols[seg * C + c] = sum;
-  __syncthreads();
-  if ((int)threadIdx.x < C) {  // each thread adds up its own column only
-    uint64_t tot = 0;
+  // S segments of RW columns
 #pragma unroll
-    for (int q = 0; q < S; ++q) tot += s_cols[q * C + threadIdx.x];
-    s_cols[threadIdx.x] = tot;
+  for (int w = 0; w < 4; ++w) s_cols[seg * RW + 4 * q + w] = sum[w];
+  __syncthreads();
+  // every thread adds four segments of one column (S * RW = 4 BLOCK values in all)
+  constexpr int P2 = BLOCK / RW;  // partial sums per column after this step
+  const int c = (int)threadIdx.x % RW, part = (int)threadIdx.x / RW;
+  uint64_t tot = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) tot += s_cols[(part * 4 + k) * RW + c];
+  __syncthreads();
+  s_cols[part * RW + c] = tot;
+  __syncthreads();
+  if ((int)threadIdx.x < RW) {
+    uint64_t t2 = 0;
+    for (int k = 0; k < P2; ++k) t2 += s_cols[k * RW + threadIdx.x];
+    s_cols[threadIdx.x] = t2;  // each thread rewrites its own column's first slot only
   }
   __syncthreads();
   if ((int)threadIdx.x < npts) {
@@ -940,21 +955,8 @@ __device__ __forceinline__ void reduce_partials_and_finish(const uint32_t* parti
   constexpr int L = F::L;
   const bool early = round > 0;
   if (early && threadIdx.x < L) s_claim[threadIdx.x] = st->claim[threadIdx.x];
-  if constexpr (F::BITS < 32 * L) {
-    __shared__ uint64_t s_cols[BLOCK];
-    sum_partials_cols<F, PMAX>(partials, nblocks, npts, s_cols, s_g);  // its barriers publish s_claim
-  } else {
-    Elem<L> acc[PMAX];
-#pragma unroll
-    for (int t = 0; t < PMAX; ++t) acc[t] = zero<F>();
-    for (uint32_t b = threadIdx.x; b < nblocks; b += BLOCK) {
-#pragma unroll
-      for (int t = 0; t < PMAX; ++t) {
-        if (t < npts) acc[t] = add<F>(acc[t], load_elem<L>(partials + ((uint64_t)b * MAX_POINTS + t) * L));
-      }
-    }
-    block_reduce<F, PMAX, BLOCK>(acc, npts, s_red, s_g);  // its barriers publish s_claim too
-  }
+  __shared__ uint64_t s_cols[4 * BLOCK];
+  sum_partials_cols<F, PMAX>(partials, nblocks, npts, s_cols, s_g);  // its barriers publish s_claim
   if (scale_pow != 0) {  // the pass summed g / 32^scale_pow
     if (threadIdx.x < npts) {
       const Elem<L> g = mont_mul<F>(load_elem<L>(s_g + threadIdx.x * L), load_elem<L>(fc->pow32[scale_pow]));

@@ -215,20 +215,24 @@ extern "C" void r29_fold_tab_host(int mont, uint64_t n, const uint32_t* lo, cons
   }
 }
 
-// V mod p from L word-column sums per value (reduce_word_columns), 128- and 256-bit fields.
-extern "C" void word_columns_host(int bit_width, uint64_t n, const uint64_t* cols, uint32_t* out) {
+// V mod p from L word-column sums per value (reduce_word_columns).
+template <class F>
+static void word_columns(uint64_t n, const uint64_t* cols, uint32_t* out) {
+  constexpr int L = F::L;
   for (uint64_t i = 0; i < n; ++i) {
-    if (bit_width == 128) {
-      uint64_t c[4];
-      for (int k = 0; k < 4; ++k) c[k] = cols[i * 4 + k];
-      const Elem<4> x = reduce_word_columns<FieldP128>(c);
-      for (int k = 0; k < 4; ++k) out[i * 4 + k] = x.v[k];
-    } else {
-      uint64_t c[8];
-      for (int k = 0; k < 8; ++k) c[k] = cols[i * 8 + k];
-      const Elem<8> x = reduce_word_columns<FieldP256>(c);
-      for (int k = 0; k < 8; ++k) out[i * 8 + k] = x.v[k];
-    }
+    uint64_t c[L];
+    for (int k = 0; k < L; ++k) c[k] = cols[i * L + k];
+    const Elem<L> x = reduce_word_columns<F>(c);
+    for (int k = 0; k < L; ++k) out[i * L + k] = x.v[k];
+  }
+}
+
+extern "C" void word_columns_host(int bit_width, uint64_t n, const uint64_t* cols, uint32_t* out) {
+  switch (bit_width) {
+    case 32: word_columns<FieldP32>(n, cols, out); break;
+    case 64: word_columns<FieldP64>(n, cols, out); break;
+    case 128: word_columns<FieldP128>(n, cols, out); break;
+    default: word_columns<FieldP256>(n, cols, out); break;
   }
 }
 

@@ -111,7 +111,7 @@ def harness(tmp_path_factory):
     op.fold_tab = fold_tab
 
     def word_columns(bw, cols):
-        L = 4 if bw == 128 else 8
+        L = get_field(bw).limbs
         c = np.ascontiguousarray(cols, dtype=np.uint64).reshape(-1, L)
         out = np.zeros(c.shape, dtype=np.uint32)
         lib.word_columns_host(bw, c.shape[0], c.ctypes.data_as(u64p), out.ctypes.data_as(u32p))
@@ -282,7 +282,7 @@ def test_host_r29_fold_by_table(harness):
     assert W > top
 
 
-@pytest.mark.parametrize("bw", [128, 256])
+@pytest.mark.parametrize("bw", [32, 64, 128, 256])
 def test_host_word_column_sums(harness, bw):
     """Sums of canonical values added word by word into 64-bit columns (no carries), then
     reduced once (reduce_word_columns, the block-partial sums of the 8-limb finalize): equal to
@@ -295,11 +295,13 @@ def test_host_word_column_sums(harness, bw):
     for n in [1, 2, 3, 255, 256, 4096]:
         cases.append([rng.randrange(p) for _ in range(n)])
     cases += [[p - 1] * 4096, [0] * 17, [p - 1] * 3, [1] * 5]
-    # sums that land on or next to a multiple of p
+    # sums that land on or next to a multiple of p, and just past one by less than p 2^-20 (the
+    # quotient is then one short and V - q p exceeds p: past 2^(32L) at 32 and 64 bits)
     for k in [1, 2, 1000, 4095]:
         xs = [rng.randrange(p) for _ in range(k)]
         s = sum(xs)
-        for target in (((s // p) + 1) * p, ((s // p) + 1) * p - 1, ((s // p) + 1) * p + 1):
+        m = ((s // p) + 1) * p
+        for target in (m, m - 1, m + 1, m + 5, m + (1 << 33), m + (p >> 21)):
             extra = target - s
             if 0 <= extra < p:
                 cases.append(xs + [extra])
