@@ -90,6 +90,12 @@ def measure_width(ctx, bw: int, nv: int, steps: int, warmup: int, mad_rate: floa
             pr.set_schedule(fixed_schedule[0], fixed_schedule[1], False)
             pr.set_point_parallel(fixed_schedule[2])
             schedule = {"max_pass_blocks": fixed_schedule[0], "tail_pairs": fixed_schedule[1]}
+            # without the autotuner's timings first, the device would start the timed proofs
+            # cold: an idle MI355X ran the first ~0.1 s of proofs 10-15% slow
+            t0 = time.perf_counter()
+            while time.perf_counter() - t0 < 0.3:
+                pr.run("sha3")
+                pr.wait()
         else:
             schedule = tune_proof(pr)  # kernel choice, grid cap and tail depth timed on this shape
         # rounds at or below this pair count run fold + point-parallel evaluation, not pass_kernel
