@@ -946,18 +946,19 @@ __device__ __forceinline__ void sum_partials_cols(const uint32_t* __restrict__ p
   static_assert(RW % 4 == 0 && BLOCK % TPR == 0, "16-byte loads over whole rows");
   const int q = (int)threadIdx.x % TPR, seg = (int)threadIdx.x / TPR;
   uint64_t sum[4] = {0, 0, 0, 0};
-  // 16 rows per thread per batch: one memory round trip for up to 16 S partials (the partials
-  // were written by every XCD, so they come from memory, not this XCD's L2)
-  for (uint32_t b0 = (uint32_t)seg; b0 < nblocks; b0 += 16u * S) {
-    u32x4_t x[16];
+  // 8 rows per thread per batch (16 measured slower: the Goldilocks pass kernel, which inlines
+  // this for its last-block finish, went from 95 registers and 5 waves per SIMD to 142 and 3;
+  // 2.27 -> 2.47 ms per proof, same box)
+  for (uint32_t b0 = (uint32_t)seg; b0 < nblocks; b0 += 8u * S) {
+    u32x4_t x[8];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
+    for (int u = 0; u < 8; ++u) {
       const uint32_t b = b0 + (uint32_t)u * S;
       x[u] = b < nblocks ? reinterpret_cast<const u32x4_t*>(partials + (uint64_t)b * MAX_POINTS * L)[q]
                          : u32x4_t{0u, 0u, 0u, 0u};
     }
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
+    for (int u = 0; u < 8; ++u) {
       sum[0] += x[u].x;
       sum[1] += x[u].y;
       sum[2] += x[u].z;

@@ -311,8 +311,9 @@ namespace {
 int default_pass_blocks(int L) {
   switch (L) {
     case 1: return 1024;  // measured ~10% faster than 2048 at num_vars 26
-    case 2: return 1024;
-    case 4: return 1024;  // the autotuner's pick at num_vars 24 (512-1024 beat 2048 by 2-3%)
+    case 2: return 512;   // the autotuner's pick since the finalize reads partials as word
+                          // columns (num_vars 26: 2.26-2.27 ms at 1024 blocks, 2.19-2.21 at 512)
+    case 4: return 512;   // the autotuner's pick at num_vars 24 (1.41 ms at 1024, 1.38 at 512)
     default: return 512;
   }
 }
