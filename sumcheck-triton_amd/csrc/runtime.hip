@@ -310,10 +310,9 @@ namespace {
 // workload: 22.9 ms at 512 blocks, 19.4 ms at 4096).
 int default_pass_blocks(int L) {
   switch (L) {
-    case 1: return 1024;  // measured ~10% faster than 2048 at num_vars 26
-    case 2: return 512;   // the autotuner's pick since the finalize reads partials as word
-                          // columns (num_vars 26: 2.26-2.27 ms at 1024 blocks, 2.19-2.21 at 512)
-    case 4: return 512;   // the autotuner's pick at num_vars 24 (1.41 ms at 1024, 1.38 at 512)
+    case 1: return 768;   // num_vars 26 sweep: 1.245 ms at 768, 1.27 at 512, 1.28 at 1024
+    case 2: return 512;   // num_vars 26: 2.21 ms at 512, 2.25 at 768, 2.29 at 1024, 3.10 at 256
+    case 4: return 512;   // num_vars 24: 1.38 ms at 512 (1.37 at 256), 1.40 at 768, 1.42 at 1024
     default: return 512;
   }
 }

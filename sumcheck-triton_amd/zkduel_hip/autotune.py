@@ -32,7 +32,7 @@ from typing import Iterable
 
 from .prover import Batch, Proof
 
-PASS_BLOCKS = (512, 1024, 2048, 4096)
+PASS_BLOCKS = (512, 768, 1024, 2048, 4096)
 TAIL_PAIRS = (64, 256, 512, 2048)
 WIDE_PAIRS = (-1, 0, 1 << 15, 1 << 17, 1 << 19, 1 << 21, 1 << 40)
 
