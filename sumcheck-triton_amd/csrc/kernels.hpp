@@ -903,23 +903,21 @@ __device__ __forceinline__ void finish_round(uint32_t* g, int npts, int round, i
 // needs only the previous round's polynomial and challenge, so the round's pass (or
 // point-parallel evaluation) kernel computes it, off the path of the finalize, which reads it
 // (a later launch on the same stream): wave 0 of one extra block at the end of the grid
-// (claim_block) that does nothing else, so no evaluating block is held up by it (as block 0
-// computing it first would hold up a kernel of equal blocks by its length); with the pass
-// kernel's own last-block finish there is no extra block, and block 0 computes it first (the
-// thread that stores it releases it with its partial). Returns true for the extra block,
-// which then leaves.
+// (claim_block) that evaluates nothing, so no evaluating block is held up by it (as block 0
+// computing it first would hold up a kernel of equal blocks by its length). The kernels branch
+// to it first and the claim block never rejoins the evaluating blocks' path: a store to the
+// proof state on (or merging into) that path keeps the compiler from reading the composition
+// descriptor through the scalar cache (the constraint workload's evaluation then issued 50%
+// more vector loads).
 template <class F>
-__device__ __forceinline__ bool store_claim(ProofState* st, int round, int npts, const FieldConsts* __restrict__ fc,
-                                            int claim_block) {
-  const bool extra = (int)blockIdx.x == claim_block;
-  if (round > 0 && (extra || (claim_block < 0 && blockIdx.x == 0)) && threadIdx.x < 64) {
+__device__ __forceinline__ void store_claim(ProofState* st, int round, int npts, const FieldConsts* __restrict__ fc) {
+  if (round > 0 && threadIdx.x < 64) {
     const Elem<F::L> c = previous_claim<F>(st, round, npts, fc);
     if (threadIdx.x == 0) {
 #pragma unroll
       for (int k = 0; k < F::L; ++k) st->claim[k] = c.v[k];
     }
   }
-  return extra;
 }
 
 // Blocks of a pass or point-parallel evaluation grid that evaluate (all but the claim block).
@@ -1054,6 +1052,29 @@ struct AccLds256 {
   }
 };
 
+// A pass kernel's own finish of the round (fused_finalize): the block that increments the
+// counter last sees every partial and the claim (each block's thread 0 released its stores
+// with the fence before its increment; the claim block's thread 0 stored the claim) and sums
+// the partials and derives the challenge.
+template <class F, int PMAX, int SID>
+__device__ __forceinline__ void pass_fused_finish(const PassArgs& a, int npts, uint32_t* s_red, uint32_t* s_g,
+                                                  uint32_t* s_msg, uint32_t* s_r, uint32_t* s_claim, int* s_last) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned prev = atomicAdd(&a.st->counter, 1u);
+    *s_last = prev == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!*s_last) return;
+  __threadfence();  // acquire the other blocks' partials
+  constexpr int kScale = (F::L == 8 && SID != 0) ? static_round0_scale(SID) : 0;
+  reduce_partials_and_finish<F, PMAX>(a.partials, evaluating_blocks(a.claim_block), npts, a.round, a.num_rounds,
+                                      a.challenge_mode, a.fc, a.st, a.explicit_r, s_red, s_g, s_msg, s_r, s_claim,
+                                      a.round == 0 ? kScale : 0);
+  if (threadIdx.x == 0) a.st->counter = 0u;  // ready for the next pass
+}
+
 // Waves per SIMD the compiler must leave room for: two for the compiled 255-bit compositions
 // (the a*b*c + d*e kernel holds 227 registers per lane; left free, the compiler may spend more
 // and fall to one wave per SIMD).
@@ -1077,8 +1098,13 @@ pass_kernel(PassArgs a) {
   const int K = Shape<SID>::terms(cd);
   const int npts = Shape<SID>::npts(cd);
   const bool fold = a.fold != 0;
-  if (store_claim<F>(a.st, a.round, npts, a.fc, a.claim_block)) return;
-
+  if ((int)blockIdx.x == a.claim_block) {
+    store_claim<F>(a.st, a.round, npts, a.fc);
+    if (a.fused_finalize) {
+      pass_fused_finish<F, PMAX, SID>(a, npts, s_red, s_g, s_msg, s_r, s_claim, &s_last);
+    }
+    return;
+  }
   using W = Work<F>;
   typename W::Acc acc[PMAX];
 #pragma unroll
@@ -1195,22 +1221,7 @@ pass_kernel(PassArgs a) {
     a.partials[(uint64_t)blockIdx.x * (MAX_POINTS * L) + threadIdx.x] = s_g[threadIdx.x];
     if (a.fused_finalize) __threadfence();  // release this block's partial device-wide
   }
-  if (!a.fused_finalize) return;
-  // Last-block-done: the block that increments the counter last sees every partial.
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned prev = atomicAdd(&a.st->counter, 1u);
-    s_last = prev == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();  // acquire the other blocks' partials
-  constexpr int kScale = (F::L == 8 && SID != 0) ? static_round0_scale(SID) : 0;
-  reduce_partials_and_finish<F, PMAX>(a.partials, gridDim.x, npts, a.round, a.num_rounds, a.challenge_mode,
-                                      a.fc, a.st, a.explicit_r, s_red, s_g, s_msg, s_r, s_claim,
-                                      a.round == 0 ? kScale : 0);
-  if (threadIdx.x == 0) a.st->counter = 0u;  // ready for the next pass
+  if (a.fused_finalize) pass_fused_finish<F, PMAX, SID>(a, npts, s_red, s_g, s_msg, s_r, s_claim, &s_last);
 }
 
 template <class F, int PMAX>
@@ -1515,7 +1526,10 @@ __global__ void __launch_bounds__(BLOCK) wide_fold_kernel(PassArgs a) {
 template <class F, int TMAX, int G>
 __global__ void __launch_bounds__(BLOCK) wide_eval_kernel(PassArgs a) {
   __shared__ uint32_t s_red[BLOCK * F::L];
-  if (store_claim<F>(a.st, a.round, a.comp->npts, a.fc, a.claim_block)) return;
+  if ((int)blockIdx.x == a.claim_block) {
+    store_claim<F>(a.st, a.round, a.comp->npts, a.fc);
+    return;
+  }
   const uint64_t pairs_here = wide_pairs_per_block(a.comp->npts, a.round);
   const TableMap sm = a.round == 0 ? strided_map(a.src_stride) : fold_map(a.src_n, a.comp->num_tables, a.chunk_shift);
   wide_eval_block<F, TMAX>(a.src, sm, a.round == 0, a.round, (uint64_t)blockIdx.x * pairs_here, a.pairs,

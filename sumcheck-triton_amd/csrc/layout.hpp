@@ -190,7 +190,7 @@ struct PassArgs {
   uint32_t* partials;         // [gridDim][MAX_POINTS][L]
   const uint32_t* explicit_r; // [num_rounds-1][L] canonical, CHALLENGE_EXPLICIT only
   // Block that only computes the round's claim into st->claim (the grid's last, one past the
-  // blocks that evaluate), or -1: then block 0 computes it before its own work.
+  // blocks that evaluate; from round 1 on), or -1.
   int32_t claim_block = -1;
   int32_t pad1 = 0;
 };

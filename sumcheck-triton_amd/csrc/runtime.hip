@@ -508,7 +508,7 @@ int enqueue_rounds(sc_proof* p, int mode) {
       e.claim_block = j > 0 ? (int32_t)g : -1;
       SC_HIP(p->ln->wide_eval(s, p->tmax, p->pmax, e, (unsigned)(g + (j > 0 ? 1 : 0))));
     } else {
-      a.claim_block = (j > 0 && !p->fused_finalize) ? (int32_t)g : -1;
+      a.claim_block = j > 0 ? (int32_t)g : -1;
       SC_HIP(p->ln->pass(s, p->sid, p->tmax, p->pmax, a, (unsigned)(g + (a.claim_block >= 0 ? 1 : 0))));
     }
     if (p->profiling) SC_HIP(hipEventRecord(p->prof_ev[2 * j + 1], s));
