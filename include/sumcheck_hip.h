@@ -265,13 +265,18 @@ int sc_batch_destroy(sc_batch* batch);
  * grouped by width; tables and transcript state stay on the device between steps. A request's
  * result is the same bits as sc_prove on it. Results are read once (sc_server_result frees
  * the request). Poll states: 0 queued, 1 running, 2 done. sc_server_info: arena_used = bytes
- * of the admitted requests, arena_size = device bytes the server holds. Calls on one server
- * must be serialized. */
+ * of the admitted requests, arena_size = device bytes the server holds. Each server runs on a
+ * stream of its own, so two servers on one context (say one for short requests, one for long
+ * ones) run side by side on the GPU. Calls on one server must be serialized. */
 typedef struct sc_server sc_server;
 int sc_server_create(sc_context* ctx, uint64_t hbm_budget_bytes, int32_t max_active, sc_server** out);
 int sc_server_submit(sc_server* server, int32_t bit_width, const sc_composition* comp, int32_t num_vars,
                      const sc_input* input, int32_t challenge_mode, int64_t* handle);
 int sc_server_step(sc_server* server, int32_t* n_running, int32_t* n_queued);
+/* sc_server_step that never waits (no counterpart in the reference; for a host loop driving
+ * several servers): with the server's in-flight steps (3) already enqueued it enqueues
+ * nothing and sets *stepped = 0; otherwise it steps as sc_server_step and sets *stepped = 1. */
+int sc_server_try_step(sc_server* server, int32_t* n_running, int32_t* n_queued, int32_t* stepped);
 int sc_server_poll(sc_server* server, int64_t handle, int32_t* state);
 /* Schedule of a queued or running request, as sc_batch_set_schedules for one instance (tile and
  * point-parallel threshold; the tail threshold is the default). A request's largest round

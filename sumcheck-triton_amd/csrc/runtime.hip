@@ -1844,6 +1844,7 @@ struct StepRecord {
 
 struct sc_server {
   sc_context* ctx = nullptr;
+  hipStream_t stream = nullptr;  // the server's own: two servers on one device run side by side
   int max_active = 0;
   uint32_t items = 4;
   ChunkArena arena;
@@ -1875,7 +1876,7 @@ struct sc_server {
 namespace {
 
 void free_server(sc_server* sv) {
-  if (sv->ctx) (void)hipStreamSynchronize(sv->ctx->stream);
+  if (sv->stream) (void)hipStreamSynchronize(sv->stream);
   for (auto& kv : sv->reqs)
     if (kv.second.result) (void)hipHostFree(kv.second.result);
   for (auto& st : sv->staging) {
@@ -1892,6 +1893,7 @@ void free_server(sc_server* sv) {
   if (sv->d_states) (void)hipFree(sv->d_states);
   if (sv->d_entries) (void)hipFree(sv->d_entries);
   if (sv->d_partials) (void)hipFree(sv->d_partials);
+  if (sv->stream) (void)hipStreamDestroy(sv->stream);
   delete sv;
 }
 
@@ -1996,12 +1998,16 @@ int sc_server_create(sc_context* ctx, uint64_t hbm_budget_bytes, int32_t max_act
     }                                                                                         \
   } while (0)
   sv->arena.budget = hbm_budget_bytes;
+  SC_HIP_S(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
   SC_HIP_S(hipMalloc(&sv->d_insts, sizeof(InstDesc) * max_active));
   SC_HIP_S(hipMalloc(&sv->d_comps, sizeof(CompDesc) * max_active));
   SC_HIP_S(hipMalloc(&sv->d_states, sizeof(ProofState) * max_active));
   SC_HIP_S(hipMalloc(&sv->d_entries, sizeof(StageEntry) * 2 * max_active));
   SC_HIP_S(hipMalloc(&sv->d_partials, sv->max_tiles * MAX_POINTS * MAX_LIMBS * 4));
-  sv->staging.resize(4);
+  // a step takes one buffer for its admissions and one per width; with 3 steps in flight,
+  // 16 buffers keep the host from waiting on a buffer that is still queued behind earlier
+  // steps' work (a wait that would also hold up any other server the host loop drives)
+  sv->staging.resize(16);
   for (auto& st : sv->staging) {
     SC_HIP_S(hipHostMalloc(&st.insts, sizeof(InstDesc) * max_active, hipHostMallocDefault));
     SC_HIP_S(hipHostMalloc(&st.comps, sizeof(CompDesc) * max_active, hipHostMallocDefault));
@@ -2047,21 +2053,39 @@ int sc_server_submit(sc_server* sv, int32_t bit_width, const sc_composition* com
   return SC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
 // One scheduling step: retire finished steps, admit queued requests that fit (FIFO), and
-// enqueue one round of every running request (batched stage launches per field width).
-int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
+// enqueue one round of every running request (batched stage launches per field width). With
+// max_inflight steps already enqueued it waits for the oldest, or (wait = false) returns with
+// *stepped = 0 and enqueues nothing.
+int server_step(sc_server* sv, bool wait, int32_t* n_running, int32_t* n_queued, int32_t* stepped) {
   if (sv == nullptr) return fail(SC_ERR_ARG, "server must not be NULL");
   sc_context* ctx = sv->ctx;
   SC_HIP(hipSetDevice(ctx->device));
+  if (stepped) *stepped = 0;
   int rc = retire(sv, false);
   if (rc != SC_OK) return rc;
   while (sv->inflight.size() >= sv->max_inflight) {
+    if (!wait) {
+      int running = 0;
+      for (int slot = 0; slot < sv->max_active; ++slot)
+        if (sv->slot_req[slot] != 0) ++running;
+      if (n_running) *n_running = running;
+      if (n_queued) *n_queued = (int32_t)sv->queue.size();
+      return SC_OK;
+    }
     SC_HIP(hipEventSynchronize(sv->inflight.front().ev));
     rc = retire(sv, false);
     if (rc != SC_OK) return rc;
   }
-  hipStream_t s = ctx->stream;
-  // admit
+  if (stepped) *stepped = 1;
+  hipStream_t s = sv->stream;
+  // admit: the step's admissions share one staging buffer, entry k for the k-th request
+  int adm_si = -1;
+  int adm_k = 0;
   while (!sv->queue.empty() && !sv->free_slots.empty()) {
     Request& r = sv->reqs[sv->queue.front()];
     const uint64_t bytes = request_bytes(r);
@@ -2091,10 +2115,13 @@ int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
     const uint64_t n0 = (uint64_t)1 << r.num_vars;
     const uint64_t words = (uint64_t)r.desc.num_tables * n0 * r.L;
     auto align = [](uint64_t x) { return (x + 255) & ~(uint64_t)255; };
-    const int si = take_staging(sv);
-    if (si < 0) return fail(SC_ERR_HIP, "no staging buffer");
-    auto& st = sv->staging[si];
-    InstDesc& d = st.insts[0];
+    if (adm_si < 0) {
+      adm_si = take_staging(sv);
+      if (adm_si < 0) return fail(SC_ERR_HIP, "no staging buffer");
+    }
+    auto& st = sv->staging[adm_si];
+    const int k = adm_k++;  // at most max_active admissions per step: one per free slot
+    InstDesc& d = st.insts[k];
     d.input = (const uint32_t*)base;
     d.buf_a = (uint32_t*)(base + align(words * 4));
     d.buf_b = (uint32_t*)(base + align(words * 4) + align(words / 2 * 4));
@@ -2103,9 +2130,9 @@ int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
     d.challenge_mode = r.mode;
     d.comp = sv->d_comps + r.slot;
     d.st = sv->d_states + r.slot;
-    st.comps[0] = r.desc;
-    SC_HIP(hipMemcpyAsync(sv->d_insts + r.slot, &st.insts[0], sizeof(InstDesc), hipMemcpyHostToDevice, s));
-    SC_HIP(hipMemcpyAsync(sv->d_comps + r.slot, &st.comps[0], sizeof(CompDesc), hipMemcpyHostToDevice, s));
+    st.comps[k] = r.desc;
+    SC_HIP(hipMemcpyAsync(sv->d_insts + r.slot, &st.insts[k], sizeof(InstDesc), hipMemcpyHostToDevice, s));
+    SC_HIP(hipMemcpyAsync(sv->d_comps + r.slot, &st.comps[k], sizeof(CompDesc), hipMemcpyHostToDevice, s));
     SC_HIP(hipMemsetAsync(sv->d_states + r.slot, 0, sizeof(ProofState), s));
     SC_HIP(hipEventRecord(st.ev, s));
     st.busy = true;
@@ -2217,6 +2244,18 @@ int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
   if (n_running) *n_running = running;
   if (n_queued) *n_queued = (int32_t)sv->queue.size();
   return SC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sc_server_step(sc_server* sv, int32_t* n_running, int32_t* n_queued) {
+  return server_step(sv, true, n_running, n_queued, nullptr);
+}
+
+int sc_server_try_step(sc_server* sv, int32_t* n_running, int32_t* n_queued, int32_t* stepped) {
+  return server_step(sv, false, n_running, n_queued, stepped);
 }
 
 int sc_server_poll(sc_server* sv, int64_t handle, int32_t* state) {

@@ -79,6 +79,14 @@ class Server:
         _lib.check(self.lib.sc_server_step(self.handle, ctypes.byref(running), ctypes.byref(queued)))
         return running.value, queued.value
 
+    def try_step(self) -> tuple[int, int, bool]:
+        """step() that never waits: with the server's in-flight steps already enqueued it
+        enqueues nothing. Returns (running, queued, stepped)."""
+        running, queued, stepped = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _lib.check(self.lib.sc_server_try_step(self.handle, ctypes.byref(running), ctypes.byref(queued),
+                                               ctypes.byref(stepped)))
+        return running.value, queued.value, bool(stepped.value)
+
     def poll(self, handle: int) -> str:
         st = ctypes.c_int32()
         _lib.check(self.lib.sc_server_poll(self.handle, int(handle), ctypes.byref(st)))
@@ -103,19 +111,83 @@ class Server:
         return {"arena_used": used.value, "arena_size": size.value, "steps": steps.value}
 
 
+SHORT_WORDS = 1 << 24  # input words (tables x 2^num_vars x limbs) of a short request: 64 MiB
+
+
+def request_words(bit_width: int, comp: Composition, num_vars: int) -> int:
+    """Input size of a request in 32-bit words: what its first round reads."""
+    return comp.num_tables * (1 << int(num_vars)) * get_field(bit_width).limbs
+
+
 def run_poisson(ctx: Context, requests: Sequence[dict], hbm_budget_bytes: int, max_active: int = 256,
-                challenge_mode: str = "sha3", schedules: dict | None = None) -> dict:
+                challenge_mode: str = "sha3", schedules: dict | None = None, lanes: int = 1,
+                short_words: int = SHORT_WORDS, warm: bool = False) -> dict:
     """Replay ``requests`` (dicts with arrival, bit_width, num_vars, composition, seed,
-    instance; see synthetic.serving_requests) against a Server in real time. Returns
-    throughput, latency percentiles and every request's checksum."""
+    instance; see synthetic.serving_requests) against a server in real time. Returns
+    throughput, latency percentiles, every request's latency (ms, by request id) and every
+    request's checksum.
+
+    lanes = 2 runs two servers side by side on the device, each on its own stream: requests
+    of at most ``short_words`` input words go to the short lane (an eighth of the HBM budget),
+    the rest to the long lane, so a short request's rounds do not queue behind a long
+    request's. The host loop steps each lane without waiting (Server.try_step) whenever it
+    has room. warm = True first runs the whole request stream once, all submitted at once and
+    untimed, on the same servers: the timed replay then sees a server whose device memory is
+    already allocated, as a long-running one is."""
+    if lanes not in (1, 2):
+        raise ValueError("lanes must be 1 or 2")
     reqs = sorted(requests, key=lambda r: r["arrival"])
     checksums: dict[int, int] = {}
     latency: dict[int, float] = {}
     t_setup = time.perf_counter()
-    sv = Server(ctx, hbm_budget_bytes, max_active, schedules)
+    if lanes == 1:
+        servers = [Server(ctx, hbm_budget_bytes, max_active, schedules)]
+    else:
+        short_budget = hbm_budget_bytes // 8
+        servers = [Server(ctx, short_budget, max_active, schedules),
+                   Server(ctx, hbm_budget_bytes - short_budget, max_active, schedules)]
     setup_s = time.perf_counter() - t_setup
+
+    def lane_of(r: dict) -> int:
+        if lanes == 1:
+            return 0
+        return 0 if request_words(r["bit_width"], r["composition"], r["num_vars"]) <= short_words else 1
+
+    def submit(r: dict) -> tuple[int, int]:
+        k = lane_of(r)
+        return k, servers[k].submit(r["bit_width"], r["composition"], r["num_vars"],
+                                    {"kind": "uniform", "seed": r["seed"], "instance": r["instance"]},
+                                    challenge_mode)
+
+    def step_all(live_lanes: set) -> int:
+        if lanes == 1:
+            return servers[0].step()[0]
+        running, stepped = 0, False
+        for k in live_lanes:
+            n, _q, did = servers[k].try_step()
+            running += n
+            stepped = stepped or did
+        if not stepped:
+            time.sleep(2e-5)
+        return running
+
     try:
-        live: dict[int, dict] = {}
+        warm_s = 0.0
+        if warm and reqs:
+            w0 = time.perf_counter()
+            pending = [submit(r) for r in reqs]
+            while pending:
+                step_all({k for k, _ in pending})
+                still = []
+                for k, h in pending:
+                    if servers[k].poll(h) == "done":
+                        servers[k].result(h)
+                    else:
+                        still.append((k, h))
+                pending = still
+            warm_s = time.perf_counter() - w0
+        steps0 = sum(sv.info()["steps"] for sv in servers)
+        live: dict[tuple[int, int], dict] = {}
         nxt = 0
         t0 = time.perf_counter()
         peak_running = 0
@@ -123,25 +195,24 @@ def run_poisson(ctx: Context, requests: Sequence[dict], hbm_budget_bytes: int, m
             now = time.perf_counter() - t0
             while nxt < len(reqs) and reqs[nxt]["arrival"] <= now:
                 r = reqs[nxt]
-                h = sv.submit(r["bit_width"], r["composition"], r["num_vars"],
-                              {"kind": "uniform", "seed": r["seed"], "instance": r["instance"]}, challenge_mode)
-                live[h] = r
+                live[submit(r)] = r
                 nxt += 1
             if live:
-                running, _queued = sv.step()
+                running = step_all({k for k, _ in live})
                 peak_running = max(peak_running, running)
                 now = time.perf_counter() - t0
-                for h in [h for h in live if sv.poll(h) == "done"]:
-                    r = live.pop(h)
+                for key in [key for key in live if servers[key[0]].poll(key[1]) == "done"]:
+                    r = live.pop(key)
                     latency[r["id"]] = now - r["arrival"]
-                    checksums[r["id"]] = sv.result(h)[2]
+                    checksums[r["id"]] = servers[key[0]].result(key[1])[2]
             elif nxt < len(reqs):
                 time.sleep(max(0.0, min(reqs[nxt]["arrival"] - now, 0.001)))
         total = time.perf_counter() - t0
-        steps = sv.info()["steps"]
+        steps = sum(sv.info()["steps"] for sv in servers) - steps0
     finally:
         t_down = time.perf_counter()
-        sv.close()
+        for sv in servers:
+            sv.close()
         teardown_s = time.perf_counter() - t_down
     lat = sorted(latency.values())
     return {
@@ -153,7 +224,11 @@ def run_poisson(ctx: Context, requests: Sequence[dict], hbm_budget_bytes: int, m
         "p99_ms": 1000.0 * lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))],
         "peak_running": peak_running,
         "steps": steps,
+        "lanes": lanes,
+        "warm": warm,
+        "warm_s": warm_s,
         "server_setup_s": setup_s,
         "server_teardown_s": teardown_s,
+        "latency_ms": {i: 1000.0 * v for i, v in latency.items()},
         "checksums": checksums,
     }

@@ -78,6 +78,25 @@ def test_server_results_match_alone(ctx):
             assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]) and got[2] == want[2], (bw, nv)
 
 
+def test_two_servers_side_by_side_with_try_step(ctx):
+    # two servers on one context, each on its own stream, stepped without waiting
+    reqs = mixed_requests(16, 7)
+    with Server(ctx, 1 << 30, max_active=8) as a, Server(ctx, 1 << 30, max_active=8) as b:
+        handles = {}
+        for k, (bw, comp, nv, i) in enumerate(reqs):
+            sv = a if k % 2 else b
+            handles[(sv, sv.submit(bw, comp, nv, {"seed": 13, "instance": i}))] = (bw, comp, nv, i)
+        skipped = 0
+        while any(sv.poll(h) != "done" for sv, h in handles):
+            for sv in (a, b):
+                skipped += not sv.try_step()[2]
+        say("try_step skipped (in-flight steps full)", skipped)
+        for (sv, h), (bw, comp, nv, i) in handles.items():
+            got = sv.result(h)
+            want = alone(ctx, bw, comp, nv, 13, i)
+            assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]) and got[2] == want[2], (bw, nv)
+
+
 def test_server_budget_admits_one_at_a_time(ctx):
     comp = Composition.from_spec(get_spec("poly_abc_plus_de"))
     nv = 14  # 5 tables x 16384 x 8 B = 640 KiB input, 1.12 MiB with the fold buffers
@@ -93,11 +112,16 @@ def test_server_budget_admits_one_at_a_time(ctx):
             sv.submit(64, comp, 20, {"seed": 3, "instance": 9})
 
 
-def test_poisson_replay_results_match_alone(ctx):
+@pytest.mark.parametrize("lanes,warm", [(1, False), (2, True)])
+def test_poisson_replay_results_match_alone(ctx, lanes, warm):
+    # two lanes: requests over 2^14 input words run on the second server's stream, side by
+    # side with the short ones; warm: the stream runs once untimed on the same servers first
     reqs = serving_requests(40, rate_per_s=2000.0, seed=6, nv_lo=4, nv_hi=12)
-    stats = run_poisson(ctx, reqs, hbm_budget_bytes=1 << 31, max_active=16)
-    say("poisson", {k: v for k, v in stats.items() if k != "checksums"})
+    stats = run_poisson(ctx, reqs, hbm_budget_bytes=1 << 31, max_active=16, lanes=lanes,
+                        short_words=1 << 14, warm=warm)
+    say("poisson", {k: v for k, v in stats.items() if k not in ("checksums", "latency_ms")})
     assert stats["requests"] == 40 and len(stats["checksums"]) == 40
+    assert sorted(stats["latency_ms"]) == sorted(r["id"] for r in reqs)
     for r in reqs:
         want = alone(ctx, r["bit_width"], r["composition"], r["num_vars"], r["seed"], r["instance"])[2]
         assert stats["checksums"][r["id"]] == want
