@@ -277,6 +277,17 @@ int sc_server_step(sc_server* server, int32_t* n_running, int32_t* n_queued);
  * several servers): with the server's in-flight steps (3) already enqueued it enqueues
  * nothing and sets *stepped = 0; otherwise it steps as sc_server_step and sets *stepped = 1. */
 int sc_server_try_step(sc_server* server, int32_t* n_running, int32_t* n_queued, int32_t* stepped);
+/* Which running requests a step advances (no counterpart in the reference):
+ * SC_SERVER_ALL_RUNNING (the default) one round of every running request;
+ * SC_SERVER_SHORTEST_FIRST the request with the least work left, every request whose current
+ * round costs at most a quarter of that one's, and further requests in order of work left
+ * while the step holds about 0.25 ms of device work (work = pairs x tables x points x a
+ * per-width weight). Shortest-first lets the smaller requests of a burst finish first instead
+ * of all finishing together; under sustained overload a large request can wait long. Results
+ * do not depend on the policy. */
+#define SC_SERVER_ALL_RUNNING 0
+#define SC_SERVER_SHORTEST_FIRST 1
+int sc_server_set_policy(sc_server* server, int32_t policy);
 int sc_server_poll(sc_server* server, int64_t handle, int32_t* state);
 /* Schedule of a queued or running request, as sc_batch_set_schedules for one instance (tile and
  * point-parallel threshold; the tail threshold is the default). A request's largest round

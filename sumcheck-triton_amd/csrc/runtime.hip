@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
@@ -1871,6 +1872,7 @@ struct sc_server {
   std::deque<StepRecord> inflight;
   uint64_t steps = 0;
   size_t max_inflight = 3;  // steps enqueued ahead of the device before the host waits
+  int policy = SC_SERVER_ALL_RUNNING;
 };
 
 namespace {
@@ -1901,6 +1903,18 @@ uint64_t request_bytes(const Request& r) {
   const uint64_t words = (uint64_t)r.desc.num_tables * ((uint64_t)1 << r.num_vars) * r.L;
   auto align = [](uint64_t x) { return (x + 255) & ~(uint64_t)255; };
   return align(words * 4) + align(words / 2 * 4) + align(words / 4 * 4 + 4);
+}
+
+// Relative device time of a request's current round: pairs x tables x points x a per-width
+// weight (limbs, and 16 at 255 bits, whose products are compute-bound: MI355X, a*b*c + d*e,
+// 3.5 ms for 2^24 pairs over all rounds at 255 bits against 2.2 ms for 2^26 at 64 bits).
+// About 0.25-0.3 ms of device time in these units on MI355X (a*b*c + d*e round 0: 2.7e9 at
+// 255 bits, num_vars 24, 0.65 ms; 6.7e8 at 64 bits, num_vars 26, ~0.2 ms).
+constexpr double kStepFill = 1e9;
+
+double round_work(const Request& r) {
+  const double pairs = std::ldexp(1.0, r.num_vars - r.round - 1);
+  return pairs * r.desc.num_tables * r.npts * (r.L == 8 ? 16.0 : (double)r.L);
 }
 
 // A free staging buffer, waiting for the oldest if all are in flight.
@@ -1974,6 +1988,14 @@ int sc_server_set_request_schedule(sc_server* sv, int64_t handle, int32_t items_
   if (rc != SC_OK) return rc;
   it->second.items = r.items;
   it->second.wide = r.wide;
+  return SC_OK;
+}
+
+int sc_server_set_policy(sc_server* sv, int32_t policy) {
+  if (sv == nullptr) return fail(SC_ERR_ARG, "server must not be NULL");
+  if (policy != SC_SERVER_ALL_RUNNING && policy != SC_SERVER_SHORTEST_FIRST)
+    return fail(SC_ERR_ARG, "policy must be SC_SERVER_ALL_RUNNING or SC_SERVER_SHORTEST_FIRST");
+  sv->policy = policy;
   return SC_OK;
 }
 
@@ -2149,7 +2171,35 @@ int server_step(sc_server* sv, bool wait, int32_t* n_running, int32_t* n_queued,
     f.fc = ctx->d_fc[r.fidx];
     SC_HIP(ln.fill(s, f));
   }
-  // one round of every running request, grouped by width
+  // Shortest remaining work first: the request with the least work left; every request whose
+  // current round costs at most a quarter of that one's (they delay it little); and further
+  // requests in order of work left while the step holds at most max(kStepFill, 1.25x the
+  // first one's round), so a step led by a nearly finished request still fills the device.
+  // The rest wait. Otherwise one round of every running request.
+  std::vector<char> take(sv->max_active, 1);
+  if (sv->policy == SC_SERVER_SHORTEST_FIRST) {
+    std::vector<std::pair<double, int>> order;  // (current round's work, slot); work left ~2x
+    for (int slot = 0; slot < sv->max_active; ++slot) {
+      const int64_t id = sv->slot_req[slot];
+      take[slot] = 0;
+      if (id == 0 || sv->reqs[id].state != REQ_RUNNING) continue;
+      order.emplace_back(round_work(sv->reqs[id]), slot);
+    }
+    std::sort(order.begin(), order.end());
+    if (!order.empty()) {
+      const double lead_work = order[0].first;
+      const double budget = std::max(kStepFill, 1.25 * lead_work);
+      double acc = 0.0;
+      for (size_t k = 0; k < order.size(); ++k) {
+        const double w = order[k].first;
+        if (k == 0 || w <= 0.25 * lead_work || acc + w <= budget) {
+          take[order[k].second] = 1;
+          acc += w;
+        }
+      }
+    }
+  }
+  // one round of every request taken, grouped by width
   StepRecord rec;
   int staged = 0;
   bool any = false;
@@ -2160,7 +2210,7 @@ int server_step(sc_server* sv, bool wait, int32_t* n_running, int32_t* n_queued,
       const int64_t id = sv->slot_req[slot];
       if (id == 0) continue;
       const Request& r = sv->reqs[id];
-      if (r.state != REQ_RUNNING || r.fidx != fidx) continue;
+      if (r.state != REQ_RUNNING || r.fidx != fidx || !take[slot]) continue;
       slots.push_back(slot);
       maxT = std::max(maxT, r.desc.num_tables);
       maxP = std::max(maxP, r.npts);

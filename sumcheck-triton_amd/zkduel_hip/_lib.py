@@ -119,6 +119,7 @@ SIGNATURES = [
       ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
     ("sc_server_step", ctypes.c_int, [ctypes.c_void_p, i32p, i32p]),
     ("sc_server_try_step", ctypes.c_int, [ctypes.c_void_p, i32p, i32p, i32p]),
+    ("sc_server_set_policy", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("sc_server_poll", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, i32p]),
     ("sc_server_set_request_schedule", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64]),
     ("sc_server_drain", ctypes.c_int, [ctypes.c_void_p]),

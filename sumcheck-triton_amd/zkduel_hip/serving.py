@@ -21,6 +21,7 @@ from .fields import get_field
 from .prover import INPUT_KINDS, Composition, Context
 
 STATES = {0: "queued", 1: "running", 2: "done"}
+POLICIES = {"all": 0, "shortest_first": 1}
 
 
 class Server:
@@ -79,6 +80,13 @@ class Server:
         _lib.check(self.lib.sc_server_step(self.handle, ctypes.byref(running), ctypes.byref(queued)))
         return running.value, queued.value
 
+    def set_policy(self, policy: str) -> None:
+        """"all": each step advances every running request by one round (the default);
+        "shortest_first": the request with the least work left, every request whose current
+        round costs at most a quarter of that one's, and further requests in order of work
+        left while the step holds about 0.25 ms of device work. Results do not depend on it."""
+        _lib.check(self.lib.sc_server_set_policy(self.handle, POLICIES[policy]))
+
     def try_step(self) -> tuple[int, int, bool]:
         """step() that never waits: with the server's in-flight steps already enqueued it
         enqueues nothing. Returns (running, queued, stepped)."""
@@ -121,7 +129,7 @@ def request_words(bit_width: int, comp: Composition, num_vars: int) -> int:
 
 def run_poisson(ctx: Context, requests: Sequence[dict], hbm_budget_bytes: int, max_active: int = 256,
                 challenge_mode: str = "sha3", schedules: dict | None = None, lanes: int = 1,
-                short_words: int = SHORT_WORDS, warm: bool = False) -> dict:
+                short_words: int = SHORT_WORDS, warm: bool = False, policy: str = "all") -> dict:
     """Replay ``requests`` (dicts with arrival, bit_width, num_vars, composition, seed,
     instance; see synthetic.serving_requests) against a server in real time. Returns
     throughput, latency percentiles, every request's latency (ms, by request id) and every
@@ -131,11 +139,14 @@ def run_poisson(ctx: Context, requests: Sequence[dict], hbm_budget_bytes: int, m
     of at most ``short_words`` input words go to the short lane (an eighth of the HBM budget),
     the rest to the long lane, so a short request's rounds do not queue behind a long
     request's. The host loop steps each lane without waiting (Server.try_step) whenever it
-    has room. warm = True first runs the whole request stream once, all submitted at once and
+    has room. ``policy`` is the long-request server's (Server.set_policy; with one lane, the
+    server's); the short lane keeps "all". warm = True first runs the whole request stream once, all submitted at once and
     untimed, on the same servers: the timed replay then sees a server whose device memory is
     already allocated, as a long-running one is."""
     if lanes not in (1, 2):
         raise ValueError("lanes must be 1 or 2")
+    if policy not in POLICIES:
+        raise ValueError(f"policy must be one of {sorted(POLICIES)}")
     reqs = sorted(requests, key=lambda r: r["arrival"])
     checksums: dict[int, int] = {}
     latency: dict[int, float] = {}
@@ -146,6 +157,7 @@ def run_poisson(ctx: Context, requests: Sequence[dict], hbm_budget_bytes: int, m
         short_budget = hbm_budget_bytes // 8
         servers = [Server(ctx, short_budget, max_active, schedules),
                    Server(ctx, hbm_budget_bytes - short_budget, max_active, schedules)]
+    servers[-1].set_policy(policy)
     setup_s = time.perf_counter() - t_setup
 
     def lane_of(r: dict) -> int:
@@ -225,6 +237,7 @@ def run_poisson(ctx: Context, requests: Sequence[dict], hbm_budget_bytes: int, m
         "peak_running": peak_running,
         "steps": steps,
         "lanes": lanes,
+        "policy": policy,
         "warm": warm,
         "warm_s": warm_s,
         "server_setup_s": setup_s,

@@ -112,13 +112,15 @@ def test_server_budget_admits_one_at_a_time(ctx):
             sv.submit(64, comp, 20, {"seed": 3, "instance": 9})
 
 
-@pytest.mark.parametrize("lanes,warm", [(1, False), (2, True)])
-def test_poisson_replay_results_match_alone(ctx, lanes, warm):
+@pytest.mark.parametrize("lanes,warm,policy", [(1, False, "all"), (2, True, "all"), (1, False, "shortest_first"),
+                                               (2, False, "shortest_first")])
+def test_poisson_replay_results_match_alone(ctx, lanes, warm, policy):
     # two lanes: requests over 2^14 input words run on the second server's stream, side by
-    # side with the short ones; warm: the stream runs once untimed on the same servers first
+    # side with the short ones; warm: the stream runs once untimed on the same servers first;
+    # shortest_first: steps advance the request with the least work left first
     reqs = serving_requests(40, rate_per_s=2000.0, seed=6, nv_lo=4, nv_hi=12)
     stats = run_poisson(ctx, reqs, hbm_budget_bytes=1 << 31, max_active=16, lanes=lanes,
-                        short_words=1 << 14, warm=warm)
+                        short_words=1 << 14, warm=warm, policy=policy)
     say("poisson", {k: v for k, v in stats.items() if k not in ("checksums", "latency_ms")})
     assert stats["requests"] == 40 and len(stats["checksums"]) == 40
     assert sorted(stats["latency_ms"]) == sorted(r["id"] for r in reqs)
