@@ -1411,10 +1411,13 @@ __device__ __forceinline__ typename Work<F>::V pick_slot(const typename Work<F>:
 // C(x) by the factor lists of the descriptor: every table index a term names is wave-uniform,
 // so each factor is a scalar branch to a multiplication by a fixed register slot. A general
 // coefficient starts the product (its multiplication is one of the term's len); a +-1 term
-// starts from its first factor.
+// starts from its first factor. At 255 bits, the first factor of a term from a table in sel01
+// (round-0 entries all 0 or 1) after the product's start is applied last as a multiply by the
+// lane's small integer: selc holds it for table v in bits 4v..4v+3, plus 8.
 template <class F, int N>
 __device__ __forceinline__ void compose_slots(typename Work<F>::Acc& acc, const typename Work<F>::V (&x)[N],
-                                              const CompDesc* __restrict__ cd, int K, int min_len) {
+                                              const CompDesc* __restrict__ cd, int K, int min_len,
+                                              uint32_t sel01 = 0u, uint64_t selc = 0u) {
   using W = Work<F>;
   for (int k = 0; k < K; ++k) {
     const int len = cd->len[k];
@@ -1427,7 +1430,20 @@ __device__ __forceinline__ void compose_slots(typename Work<F>::Acc& acc, const 
       prod = pick_slot<F, N>(x, cd->fac[k][0]);
       m = 1;
     }
-    for (; m < len; ++m) mul_slot<F, N>(prod, x, cd->fac[k][m]);
+    int small = -1;  // table of the factor applied as a small integer (wave-uniform)
+    for (; m < len; ++m) {
+      const int v = cd->fac[k][m];
+      if constexpr (F::L == 8) {
+        if (small < 0 && ((sel01 >> v) & 1u)) {
+          small = v;
+          continue;
+        }
+      }
+      mul_slot<F, N>(prod, x, v);
+    }
+    if constexpr (F::L == 8) {
+      if (small >= 0) prod = W::scale_small(prod, (int)((selc >> (4 * small)) & 15u) - 8);
+    }
     if (len >= min_len) {
       if (kind == COEFF_MINUS_ONE) acc.sub(prod); else acc.add(prod);
     }
@@ -1443,7 +1459,7 @@ template <class F, int TMAX>
 __device__ __forceinline__ void wide_eval_block(const uint32_t* __restrict__ src, const TableMap& sm, bool src_mont,
                                                 int round, uint64_t begin, uint64_t end, uint64_t stride,
                                                 const CompDesc* __restrict__ cd, uint32_t* __restrict__ part,
-                                                uint32_t* s_red) {
+                                                uint32_t* s_red, uint32_t sel01 = 0u) {
   using W = Work<F>;
   constexpr int L = F::L;
   const int T = cd->num_tables;
@@ -1460,14 +1476,34 @@ __device__ __forceinline__ void wide_eval_block(const uint32_t* __restrict__ src
   const int min_len = lead ? npts - 1 : 0;
   typename W::Acc acc;
   acc.clear();
+  // Register slots while TMAX values fit in 160 VGPRs; 32 values of the 255-bit field do not,
+  // and keep the term loop (x then lives in scratch memory).
+  constexpr bool kSlots = TMAX > 8 && TMAX * (int)(sizeof(typename W::V) / 4) <= 160;
+  // 0/1 tables as small-integer factors: 255 bits, register slots, at most 16 tables
+  constexpr bool kSmall = F::L == 8 && kSlots && TMAX <= 16;
+  if constexpr (!kSmall) sel01 = 0u;
   if (group < pairs_here) {
     for (uint64_t i = begin + group; i < end; i += stride) {
       typename W::V x[TMAX];
+      uint64_t selc = 0;  // table v's value at t in bits 4v..4v+3, plus 8 (tables in sel01)
       static_for<TMAX>([&](auto vi) {
         constexpr int v = decltype(vi)::value;
         if (v < T) {
           uint32_t w[2 * L];
           load_words<2 * L>(w, src + sm.at(v, 2 * i) * L);
+          if constexpr (kSmall) {
+            if ((sel01 >> v) & 1u) {  // entries 0 or 1: any nonzero word means 1
+              uint32_t o0 = 0, o1 = 0;
+#pragma unroll
+              for (int k = 0; k < L; ++k) {
+                o0 |= w[k];
+                o1 |= w[L + k];
+              }
+              const int s0 = o0 != 0u, s1 = o1 != 0u;
+              const int c = lead ? s1 - s0 : s0 + t * (s1 - s0);  // in [-5, 6]: t <= 6
+              selc |= (uint64_t)(uint32_t)(c + 8) << (4 * v);
+            }
+          }
           const typename W::V lo = src_mont ? W::from_mont(w) : W::from_stored(w);
           const typename W::V hi = src_mont ? W::from_mont(w + L) : W::from_stored(w + L);
           const typename W::V dv = W::diff(hi, lo);
@@ -1476,11 +1512,8 @@ __device__ __forceinline__ void wide_eval_block(const uint32_t* __restrict__ src
           x[v] = x[0];  // defined (an undefined slot measured 50 more VGPRs), never read
         }
       });
-      // Register slots while TMAX values fit in 160 VGPRs; 32 values of the 255-bit field do not,
-      // and keep the term loop (x then lives in scratch memory).
-      constexpr bool kSlots = TMAX > 8 && TMAX * (int)(sizeof(typename W::V) / 4) <= 160;
       if constexpr (kSlots) {
-        compose_slots<F, TMAX>(acc, x, cd, K, min_len);
+        compose_slots<F, TMAX>(acc, x, cd, K, min_len, sel01, selc);
       } else {
         compose_add<F, TMAX>(acc, x, cd, K, min_len);
       }
@@ -1534,7 +1567,8 @@ __global__ void __launch_bounds__(BLOCK) wide_eval_kernel(PassArgs a) {
   const TableMap sm = a.round == 0 ? strided_map(a.src_stride) : fold_map(a.src_n, a.comp->num_tables, a.chunk_shift);
   wide_eval_block<F, TMAX>(a.src, sm, a.round == 0, a.round, (uint64_t)blockIdx.x * pairs_here, a.pairs,
                            (uint64_t)evaluating_blocks(a.claim_block) * pairs_here, a.comp,
-                           a.partials + (uint64_t)blockIdx.x * MAX_POINTS * F::L, s_red);
+                           a.partials + (uint64_t)blockIdx.x * MAX_POINTS * F::L, s_red,
+                           a.round == 0 ? a.sel01 : 0u);
 }
 
 // ---------------------------------------------------------------------------------------------

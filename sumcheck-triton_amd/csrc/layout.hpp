@@ -192,7 +192,9 @@ struct PassArgs {
   // Block that only computes the round's claim into st->claim (the grid's last, one past the
   // blocks that evaluate; from round 1 on), or -1.
   int32_t claim_block = -1;
-  int32_t pad1 = 0;
+  // Tables whose round-0 entries are all 0 or 1 (generated selector tables): the 255-bit
+  // point-parallel evaluation applies such a factor as a multiply by a small integer.
+  uint32_t sel01 = 0;
 };
 
 // Rounds first_round..num_rounds-1 in one workgroup. Round j >= 1 reads buf(j-1) and writes

@@ -273,6 +273,7 @@ struct sc_proof {
   uint64_t n0 = 0;
   uint32_t* d_tables = nullptr;
   uint64_t in_stride = 0;  // entries from one input table to the next (input_stride, or n0 for caller tables)
+  uint32_t sel01 = 0;      // generated selector tables (entries 0 or 1), PassArgs::sel01 of round 0
   bool owns_tables = false;
   uint32_t* d_a = nullptr;
   uint32_t* d_b = nullptr;
@@ -503,6 +504,7 @@ int enqueue_rounds(sc_proof* p, int mode) {
       e.src_stride = j == 0 ? p->in_stride : e.src_n;
       e.fold = 0;
       e.fused_finalize = 0;
+      e.sel01 = j == 0 ? p->sel01 : 0u;
       const uint64_t pairs_per_block = wide_pairs_per_block(p->npts, j);
       g = std::min<uint64_t>((a.pairs + pairs_per_block - 1) / pairs_per_block, (uint64_t)MAX_PASS_BLOCKS);
       // from round 1 one more block computes the round's claim for the finalize
@@ -857,6 +859,8 @@ int sc_proof_create(sc_context* ctx, int32_t bit_width, const sc_composition* co
     f.pad0 = 0;
     f.fc = fc;
     SC_HIP_P(p->ln->fill(s, f));
+    if (input->kind == SC_INPUT_UNIFORM)  // fill_kernel writes 0 or 1 to these tables
+      p->sel01 = (uint32_t)(input->selector_mask & (p->T >= 32 ? ~0ull : (1ull << p->T) - 1));
   }
   SC_HIP_P(hipStreamSynchronize(s));
 #undef SC_HIP_P

@@ -165,10 +165,17 @@ extern "C" void gl_host_mul_sum(uint64_t n, const uint64_t* a1, const uint64_t* 
 }
 
 // Radix-2^29 helpers of the 255-bit field on raw limbs: op 0 unpack<0> then pack<0> (identity
-// for values < 2^256); op 1 top-limb reduction of the nine limbs in a (limbs < 2^30, top < 2^32).
+// for values < 2^256); op 1 top-limb reduction of the nine limbs in a (limbs < 2^30, top < 2^32);
+// op 2 Work<FieldP256>::scale_small on rows of ten words (nine limbs, then c as int32), the
+// result's nine limbs in the row's first nine words.
 extern "C" void r29_host(int op, uint64_t n, const uint32_t* a, uint32_t* out) {
   for (uint64_t i = 0; i < n; ++i) {
-    if (op == 0) {
+    if (op == 2) {
+      R29 x;
+      for (int k = 0; k < 9; ++k) x.l[k] = a[i * 10 + k];
+      const R29 r = Work<FieldP256>::scale_small(x, (int32_t)a[i * 10 + 9]);
+      for (int k = 0; k < 9; ++k) out[i * 10 + k] = r.l[k];
+    } else if (op == 0) {
       r29_pack<0>(out + i * 8, r29_unpack<0>(a + i * 8));
     } else {
       R29 x;

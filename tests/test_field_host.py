@@ -241,6 +241,29 @@ def _words(x):
     return [(x >> (32 * k)) & 0xFFFFFFFF for k in range(8)]
 
 
+def test_host_r29_scale_small(harness):
+    """The 255-bit multiply by a small integer c (|c| <= 7), used for selector factors whose
+    round-0 values are 0 or 1: on normalized inputs up to the largest the kernels pass (top
+    limb below 2^29: products, coefficients, interpolated values up to 29p), the result is
+    congruent to c x, at most 3p (3p itself for c < 0 and x = 0), with every limb below 2^29."""
+    p = get_field(256).modulus
+    rng = random.Random(12)
+    xs = [0, 1, p - 1, p, 13 * p - 1, 29 * p + (1 << 244), (1 << 261) - 1]
+    xs += [rng.randrange(1 << 261) for _ in range(400)] + [rng.randrange(13 * p) for _ in range(400)]
+    rows = []
+    for x in xs:
+        for c in range(-7, 8):
+            rows.append([(x >> (29 * i)) & ((1 << 29) - 1) for i in range(9)] + [c & 0xFFFFFFFF])
+    out = harness.r29(2, np.array(rows, dtype=np.uint64).astype(np.uint32))
+    for row, res in zip(rows, out.tolist()):
+        x = sum(int(v) << (29 * i) for i, v in enumerate(row[:9]))
+        c = row[9] - (1 << 32) if row[9] >= 1 << 31 else row[9]
+        v = sum(int(w) << (29 * i) for i, w in enumerate(res[:9]))
+        assert all(w < (1 << 29) for w in res[:9]), (x, c)
+        assert v <= 3 * p, (x, c)
+        assert v % p == c * x % p, (x, c)
+
+
 def test_host_r29_fold_by_table(harness):
     """The 255-bit fold by the challenge's table (no Montgomery reduction; the quotient from
     the top columns in double precision) on stored values across their whole range
