@@ -273,9 +273,12 @@ int sc_server_create(sc_context* ctx, uint64_t hbm_budget_bytes, int32_t max_act
 int sc_server_submit(sc_server* server, int32_t bit_width, const sc_composition* comp, int32_t num_vars,
                      const sc_input* input, int32_t challenge_mode, int64_t* handle);
 int sc_server_step(sc_server* server, int32_t* n_running, int32_t* n_queued);
-/* sc_server_step that never waits (no counterpart in the reference; for a host loop driving
- * several servers): with the server's in-flight steps (3) already enqueued it enqueues
- * nothing and sets *stepped = 0; otherwise it steps as sc_server_step and sets *stepped = 1. */
+/* sc_server_step that does not wait for in-flight steps (no counterpart in the reference; for
+ * a host loop driving several servers): with the server's in-flight steps (3) already enqueued
+ * it enqueues nothing and sets *stepped = 0; otherwise it steps as sc_server_step and sets
+ * *stepped = 1. A step takes one of the server's 16 pinned staging buffers for its admissions
+ * and one per field width; only when all 16 are still queued (three steps in flight, each
+ * over all four widths with admissions) does it wait for the oldest to be consumed. */
 int sc_server_try_step(sc_server* server, int32_t* n_running, int32_t* n_queued, int32_t* stepped);
 /* Which running requests a step advances (no counterpart in the reference):
  * SC_SERVER_ALL_RUNNING (the default) one round of every running request;

@@ -88,8 +88,9 @@ class Server:
         _lib.check(self.lib.sc_server_set_policy(self.handle, POLICIES[policy]))
 
     def try_step(self) -> tuple[int, int, bool]:
-        """step() that never waits: with the server's in-flight steps already enqueued it
-        enqueues nothing. Returns (running, queued, stepped)."""
+        """step() that does not wait for in-flight steps: with the server's in-flight steps
+        already enqueued it enqueues nothing (sc_server_try_step). Returns (running, queued,
+        stepped)."""
         running, queued, stepped = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         _lib.check(self.lib.sc_server_try_step(self.handle, ctypes.byref(running), ctypes.byref(queued),
                                                ctypes.byref(stepped)))
@@ -138,8 +139,8 @@ def run_poisson(ctx: Context, requests: Sequence[dict], hbm_budget_bytes: int, m
     lanes = 2 runs two servers side by side on the device, each on its own stream: requests
     of at most ``short_words`` input words go to the short lane (an eighth of the HBM budget),
     the rest to the long lane, so a short request's rounds do not queue behind a long
-    request's. The host loop steps each lane without waiting (Server.try_step) whenever it
-    has room. ``policy`` is the long-request server's (Server.set_policy; with one lane, the
+    request's. The host loop steps each lane that has room (Server.try_step), without waiting
+    for either lane's in-flight steps. ``policy`` is the long-request server's (Server.set_policy; with one lane, the
     server's); the short lane keeps "all". warm = True first runs the whole request stream once, all submitted at once and
     untimed, on the same servers: the timed replay then sees a server whose device memory is
     already allocated, as a long-running one is."""
