@@ -2,9 +2,10 @@
 Poisson-arrival driver of the serving benchmark.
 
 Requests arrive over time; the server admits them first-in-first-out while a slot and HBM
-budget are free, and every step advances every running request by one round on the device
-(requests of different sizes, widths and compositions share the stage launches). Latency of
-a request = time its result is observed by the host - its arrival time.
+budget are free, and every step advances the running requests by one round each on the device
+(all of them, or with the shortest-first policy those with the least work left; requests of
+different sizes, widths and compositions share the stage launches). Latency of a request =
+time its result is observed by the host - its arrival time.
 """
 
 from __future__ import annotations
@@ -140,10 +141,11 @@ def run_poisson(ctx: Context, requests: Sequence[dict], hbm_budget_bytes: int, m
     of at most ``short_words`` input words go to the short lane (an eighth of the HBM budget),
     the rest to the long lane, so a short request's rounds do not queue behind a long
     request's. The host loop steps each lane that has room (Server.try_step), without waiting
-    for either lane's in-flight steps. ``policy`` is the long-request server's (Server.set_policy; with one lane, the
-    server's); the short lane keeps "all". warm = True first runs the whole request stream once, all submitted at once and
-    untimed, on the same servers: the timed replay then sees a server whose device memory is
-    already allocated, as a long-running one is."""
+    for either lane's in-flight steps. ``policy`` is the long-request server's step policy
+    (Server.set_policy; with one lane, the server's); the short lane keeps "all". warm = True
+    first runs the whole request stream once, all submitted at once and untimed, on the same
+    servers: the timed replay then sees a server whose device memory is already allocated, as
+    a long-running one is."""
     if lanes not in (1, 2):
         raise ValueError("lanes must be 1 or 2")
     if policy not in POLICIES:
