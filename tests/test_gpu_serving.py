@@ -129,6 +129,21 @@ def test_poisson_replay_results_match_alone(ctx, lanes, warm, policy):
         assert stats["checksums"][r["id"]] == want
 
 
+def test_two_lane_shortest_first_at_benchmark_sizes(ctx):
+    """Requests of the serving benchmark's sizes (num_vars 17-22, widths 256/64/32, random
+    product sums) on two lanes with shortest-first steps on the long one: every result equals
+    the same proof run alone."""
+    reqs = serving_requests(12, rate_per_s=500.0, seed=21, nv_lo=17, nv_hi=22)
+    stats = run_poisson(ctx, reqs, hbm_budget_bytes=64 << 30, max_active=16, lanes=2,
+                        short_words=1 << 22, policy="shortest_first")
+    say("benchmark sizes", {k: v for k, v in stats.items() if k not in ("checksums", "latency_ms")},
+        sorted((r["bit_width"], r["num_vars"]) for r in reqs))
+    assert len(stats["checksums"]) == 12
+    for r in reqs:
+        want = alone(ctx, r["bit_width"], r["composition"], r["num_vars"], r["seed"], r["instance"])[2]
+        assert stats["checksums"][r["id"]] == want, (r["bit_width"], r["num_vars"])
+
+
 def test_request_schedules_keep_results(ctx):
     reqs = mixed_requests(16, 4)
     rng = random.Random(4)
