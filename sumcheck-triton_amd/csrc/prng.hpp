@@ -36,22 +36,41 @@ SC_HD uint64_t stream_key(uint64_t seed, uint32_t instance, uint32_t table) {
 SC_HD uint64_t prng_draw(uint64_t key, uint64_t counter) { return mix64(key + counter * PRNG_GAMMA); }
 
 template <class F>
-SC_HD Elem<F::L> uniform_element(uint64_t key, uint64_t index) {
+SC_HD Elem<F::L> uniform_draw(uint64_t key, uint64_t index, int attempt) {
   constexpr int L = F::L;
   Elem<L> x;
-  for (int attempt = 0; attempt < PRNG_MAX_ATTEMPTS; ++attempt) {
 #pragma unroll
-    for (int w = 0; w < (L + 1) / 2; ++w) {
-      const uint64_t d = prng_draw(key, index * 128u + (uint64_t)attempt * 4u + (uint64_t)w);
-      x.v[2 * w] = (uint32_t)d;
-      if (2 * w + 1 < L) x.v[2 * w + 1] = (uint32_t)(d >> 32);
-    }
-    constexpr int top_bits = F::BITS - 32 * (L - 1);
-    if (top_bits < 32) x.v[L - 1] &= (1u << top_bits) - 1u;
+  for (int w = 0; w < (L + 1) / 2; ++w) {
+    const uint64_t d = prng_draw(key, index * 128u + (uint64_t)attempt * 4u + (uint64_t)w);
+    x.v[2 * w] = (uint32_t)d;
+    if (2 * w + 1 < L) x.v[2 * w + 1] = (uint32_t)(d >> 32);
+  }
+  constexpr int top_bits = F::BITS - 32 * (L - 1);
+  if (top_bits < 32) x.v[L - 1] &= (1u << top_bits) - 1u;
+  return x;
+}
+
+// The first attempt whose masked draw is below p. Each attempt draws the word holding the top
+// limb first: a top limb above p's rejects the attempt without the other draws, one below
+// accepts it, and only a tie needs the whole comparison (the same values as drawing every word
+// of every attempt, with a third of the draws at 255 bits, where 9.4% of attempts fail).
+template <class F>
+SC_HD Elem<F::L> uniform_element(uint64_t key, uint64_t index) {
+  constexpr int L = F::L;
+  constexpr int TW = (L + 1) / 2 - 1;  // the draw holding limb L - 1
+  constexpr int top_bits = F::BITS - 32 * (L - 1);
+  constexpr uint32_t top_mask = top_bits < 32 ? (1u << top_bits) - 1u : 0xFFFFFFFFu;
+  for (int attempt = 0; attempt < PRNG_MAX_ATTEMPTS; ++attempt) {
+    const uint64_t d = prng_draw(key, index * 128u + (uint64_t)attempt * 4u + (uint64_t)TW);
+    const uint32_t top = ((L & 1) ? (uint32_t)d : (uint32_t)(d >> 32)) & top_mask;
+    if (top > F::p(L - 1)) continue;  // x > p
+    const Elem<L> x = uniform_draw<F>(key, index, attempt);
+    if (top < F::p(L - 1)) return x;  // x < p
     Elem<L> t;
     if (sub_p<F>(t, x)) return x;  // x < p
   }
   // Unreachable for the four fields in practice; keep the value in range deterministically.
+  const Elem<L> x = uniform_draw<F>(key, index, PRNG_MAX_ATTEMPTS - 1);
   Elem<L> t;
   sub_p<F>(t, x);
   return t;

@@ -4,6 +4,7 @@
 
 #include "field.hpp"
 #include "keccak.hpp"
+#include "prng.hpp"
 #include "work.hpp"
 
 using namespace sc;
@@ -289,4 +290,21 @@ extern "C" void gl_host_fast(int op, uint64_t n, const uint64_t* a, const uint64
     else if (op == 4) out[i] = x.hi;
     else out[i] = (uint64_t)x.x0 | ((uint64_t)x.x1 << 32);
   }
+}
+
+// The input generator's uniform entries (prng.hpp) for indices start .. start + n - 1 of one
+// stream key: L words each.
+template <class F>
+static void uniform_run(uint64_t key, uint64_t start, uint64_t n, uint32_t* out) {
+  for (uint64_t i = 0; i < n; ++i) {
+    const Elem<F::L> x = uniform_element<F>(key, start + i);
+    for (int k = 0; k < F::L; ++k) out[i * F::L + k] = x.v[k];
+  }
+}
+
+extern "C" void uniform_host(int bit_width, uint64_t key, uint64_t start, uint64_t n, uint32_t* out) {
+  if (bit_width == 32) uniform_run<FieldP32>(key, start, n, out);
+  else if (bit_width == 64) uniform_run<FieldP64>(key, start, n, out);
+  else if (bit_width == 128) uniform_run<FieldP128>(key, start, n, out);
+  else uniform_run<FieldP256>(key, start, n, out);
 }

@@ -32,6 +32,8 @@ def harness(tmp_path_factory):
     lib.field_host_work.restype = None
     lib.r29_host.argtypes = [ctypes.c_int, ctypes.c_uint64, u32p, u32p]
     lib.r29_host.restype = None
+    lib.uniform_host.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, u32p]
+    lib.uniform_host.restype = None
     u64p = ctypes.POINTER(ctypes.c_uint64)
     lib.gl_host_mul_sum.argtypes = [ctypes.c_uint64, u64p, u64p, u64p, u64p, u64p]
     lib.gl_host_mul_sum.restype = None
@@ -122,6 +124,7 @@ def harness(tmp_path_factory):
     op.gl_mul_sum = gl_mul_sum
     op.gl_accum = gl_accum
     op.interleave = interleave
+    op.lib = lib
     return op
 
 
@@ -239,6 +242,23 @@ def test_host_r29_pack_and_reduce(harness):
 
 def _words(x):
     return [(x >> (32 * k)) & 0xFFFFFFFF for k in range(8)]
+
+
+@pytest.mark.parametrize("bw", [32, 64, 128, 256])
+def test_host_uniform_generator_matches_oracle(harness, bw):
+    """The device input generator (prng.hpp, built for the host) gives the oracle's uniform
+    entries: at 255 bits, where 9.4% of attempts are rejected, across indices with one, two
+    and three attempts."""
+    from oracle import sumcheck_ref as R
+
+    p, L = R.field(bw)
+    for seed, inst, table in ((5, 3, 2), (41, 0, 11), (1, 7, 0)):
+        key = R.stream_key(seed, inst, table)
+        n = 3000
+        out = np.zeros(n * L, dtype=np.uint32)
+        harness.lib.uniform_host(bw, key, 1000, n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+        got = [sum(int(w) << (32 * k) for k, w in enumerate(out[i * L:(i + 1) * L])) for i in range(n)]
+        assert got == [R.uniform_element(key, 1000 + i, p, L) for i in range(n)], (bw, seed)
 
 
 def test_host_r29_scale_small(harness):
